@@ -1,0 +1,231 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/librwig_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the CPU
+baseline timed beside it. Only tests/, ``__graft_entry__.smoke()`` and
+``bench.py``'s cpu_baseline leg import this module. The product package
+(``rwig_amd``) never imports it.
+
+Every function returns a dict with the element names of the reference's Rcpp
+output lists (src/sinkhorn.cpp:28-43, :154-169; src/barycenter.cpp:43-59,
+:190-206; src/wdl.cpp:48-49). Matrices are numpy arrays in column-major
+(Fortran) order, as R hands them to Rcpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librwig_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile the oracle with its Makefile (g++)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oracle_spectral_norm.restype = ctypes.c_double
+    return _lib
+
+
+def _f64(x):
+    return np.asfortranarray(np.asarray(x, dtype=np.float64))
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    return x.ctypes.data_as(_dp)
+
+
+def sinkhorn_vanilla(a, b, C, reg, withgrad=False, maxiter=1000, zerotol=1e-6):
+    """Sinkhorn::compute_vanilla (src/sinkhorn_impl_vanilla.cpp:18-86)."""
+    a, b, C = _f64(a).ravel(), _f64(b).ravel(), _f64(C)
+    M, N = C.shape
+    P = np.zeros((M, N), order="F")
+    grad = np.zeros(M)
+    u = np.zeros(M)
+    v = np.zeros(N)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    lib().oracle_sinkhorn_vanilla(
+        _ptr(a), _ptr(b), _ptr(C), M, N, ctypes.c_double(reg), int(withgrad),
+        int(maxiter), ctypes.c_double(zerotol), _ptr(P), _ptr(grad), _ptr(u),
+        _ptr(v), ctypes.byref(loss), ctypes.byref(it), ctypes.byref(err),
+        ctypes.byref(rc))
+    out = {"P": P}
+    if withgrad:
+        out["grad_a"] = grad
+    out.update(u=u, v=v, loss=loss.value, iter=it.value, err=err.value,
+               return_status=rc.value)
+    return out
+
+
+def sinkhorn_log(a, b, C, reg, withgrad=False, n_threads=0, maxiter=1000,
+                 zerotol=1e-6):
+    """Sinkhorn::compute_log (src/sinkhorn_impl_log.cpp:225-294)."""
+    a, b, C = _f64(a).ravel(), _f64(b).ravel(), _f64(C)
+    M, N = C.shape
+    P = np.zeros((M, N), order="F")
+    grad = np.zeros(M)
+    f = np.zeros(M)
+    g = np.zeros(N)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    st = lib().oracle_sinkhorn_log(
+        _ptr(a), _ptr(b), _ptr(C), M, N, ctypes.c_double(reg), int(withgrad),
+        int(n_threads), int(maxiter), ctypes.c_double(zerotol), _ptr(P),
+        _ptr(grad), _ptr(f), _ptr(g), ctypes.byref(loss), ctypes.byref(it),
+        ctypes.byref(err), ctypes.byref(rc))
+    if st != 0:
+        raise ValueError(f"oracle_sinkhorn_log failed with status {st}")
+    out = {"P": P}
+    if withgrad:
+        out["grad_a"] = grad
+    out.update(f=f, g=g, loss=loss.value, iter=it.value, err=err.value,
+               return_status=rc.value)
+    return out
+
+
+def barycenter_parallel(A, C, w, b_ext=None, reg=0.1, withgrad=False,
+                        maxiter=1000, zerotol=1e-6):
+    """Barycenter::compute_parallel (src/barycenter_impl_parallel.cpp:19-53)."""
+    A, C, w = _f64(A), _f64(C), _f64(w).ravel()
+    M, S = A.shape
+    N = C.shape[1]
+    bext = _f64(b_ext).ravel() if withgrad else np.zeros(N)
+    b = np.zeros(N)
+    U = np.zeros((M, S), order="F")
+    V = np.zeros((N, S), order="F")
+    gA = np.zeros((M, S), order="F")
+    gw = np.zeros(S)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    lib().oracle_barycenter_parallel(
+        _ptr(A), _ptr(C), _ptr(w), _ptr(bext), M, N, S, ctypes.c_double(reg),
+        int(withgrad), int(maxiter), ctypes.c_double(zerotol), _ptr(b), _ptr(U),
+        _ptr(V), _ptr(gA), _ptr(gw), ctypes.byref(loss), ctypes.byref(it),
+        ctypes.byref(err), ctypes.byref(rc))
+    out = {"b": b}
+    if withgrad:
+        out.update(grad_A=gA, grad_w=gw, loss=loss.value)
+    out.update(U=U, V=V, iter=it.value, err=err.value, return_status=rc.value)
+    return out
+
+
+def barycenter_log(A, C, w, b_ext=None, reg=0.1, withgrad=False, n_threads=0,
+                   maxiter=1000, zerotol=1e-6):
+    """Barycenter::compute_log (src/barycenter_impl_log.cpp:23-48)."""
+    A, C, w = _f64(A), _f64(C), _f64(w).ravel()
+    M, S = A.shape
+    N = C.shape[1]
+    bext = _f64(b_ext).ravel() if withgrad else np.zeros(N)
+    b = np.zeros(N)
+    F = np.zeros((M, S), order="F")
+    G = np.zeros((N, S), order="F")
+    gA = np.zeros((M, S), order="F")
+    gw = np.zeros(S)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    st = lib().oracle_barycenter_log(
+        _ptr(A), _ptr(C), _ptr(w), _ptr(bext), M, N, S, ctypes.c_double(reg),
+        int(withgrad), int(n_threads), int(maxiter), ctypes.c_double(zerotol),
+        _ptr(b), _ptr(F), _ptr(G), _ptr(gA), _ptr(gw), ctypes.byref(loss),
+        ctypes.byref(it), ctypes.byref(err), ctypes.byref(rc))
+    if st != 0:
+        raise ValueError(f"oracle_barycenter_log failed with status {st}")
+    out = {"b": b}
+    if withgrad:
+        out.update(grad_A=gA, grad_w=gw, loss=loss.value)
+    out.update(F=F, G=G, iter=it.value, err=err.value, return_status=rc.value)
+    return out
+
+
+def softmax_cols(X):
+    """Column softmax with max shift (inst/include/wdl_impl.hpp:80-96)."""
+    X = _f64(X)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    Y = np.zeros_like(X, order="F")
+    lib().oracle_softmax_cols(_ptr(X), _ptr(Y), X.shape[0], X.shape[1])
+    return Y
+
+
+def optimizer_step(opt, theta, g, m_state, v_state, step, eta=1e-3, gamma=0.01,
+                   beta1=0.9, beta2=0.999, eps=1e-8):
+    """One step of inst/include/optimizer.hpp:72-99, in place on the arrays.
+
+    Returns the new step counter."""
+    st = ctypes.c_int(step)
+    lib().oracle_optimizer_step(
+        int(opt), _ptr(theta), _ptr(_f64(g)), theta.size, _ptr(m_state),
+        _ptr(v_state), ctypes.byref(st), ctypes.c_double(eta),
+        ctypes.c_double(gamma), ctypes.c_double(beta1), ctypes.c_double(beta2),
+        ctypes.c_double(eps))
+    return st.value
+
+
+def wdl_batch_grad(K, A, Yb, Wb, L):
+    """WDL::_train_batch_batched gradient (src/wdl_impl.cpp:87-247).
+
+    Returns (g_Alpha N x S, g_lambda S, bB N x D unnormalised barycenters)."""
+    K, A, Yb, Wb = _f64(K), _f64(A), _f64(Yb), _f64(Wb)
+    N, S = A.shape
+    D = Yb.shape[1]
+    gA = np.zeros((N, S), order="F")
+    gl = np.zeros(S)
+    bB = np.zeros((N, D), order="F")
+    lib().oracle_wdl_batch_grad(_ptr(K), _ptr(A), _ptr(Yb), _ptr(Wb), N, S, D,
+                                int(L), _ptr(gA), _ptr(gl), _ptr(bB))
+    return gA, gl, bB
+
+
+def wdl(Y, C, reg, S, Alpha0, Lambda0, batch_size, epochs, sinkhorn_mode=1,
+        n_threads=0, max_iter=20, zero_tol=1e-6, optimizer=2, eta=1e-3,
+        gamma=0.01, beta1=0.9, beta2=0.999, eps=1e-8):
+    """WassersteinDictionaryLearning::_compute_serial (src/wdl_impl.cpp:254-378)
+    with explicit initial Alpha0 (N x S) and Lambda0 (S x M)."""
+    Y, C = _f64(Y), _f64(C)
+    N, M = Y.shape
+    Alpha0, Lambda0 = _f64(Alpha0), _f64(Lambda0)
+    A = np.zeros((N, S), order="F")
+    W = np.zeros((S, M), order="F")
+    Yhat = np.zeros((N, M), order="F")
+    st = lib().oracle_wdl(
+        _ptr(Y), _ptr(C), N, M, S, ctypes.c_double(reg), int(batch_size),
+        int(epochs), int(sinkhorn_mode), int(n_threads), int(max_iter),
+        ctypes.c_double(zero_tol), int(optimizer), ctypes.c_double(eta),
+        ctypes.c_double(gamma), ctypes.c_double(beta1), ctypes.c_double(beta2),
+        ctypes.c_double(eps), _ptr(Alpha0), _ptr(Lambda0), _ptr(A), _ptr(W),
+        _ptr(Yhat))
+    if st != 0:
+        raise ValueError(f"oracle_wdl failed with status {st}")
+    return {"A": A, "W": W, "Yhat": Yhat}
+
+
+def spectral_norm(X):
+    X = _f64(X)
+    if X.ndim == 1:
+        X = X.reshape(-1, 1, order="F")
+    return lib().oracle_spectral_norm(_ptr(X), X.shape[0], X.shape[1])

@@ -1,0 +1,90 @@
+/* rwig_hip.h -- C ABI of librwig_hip.so, the MI355X (gfx950) entropic-OT
+ * hot path behind rwig's sinkhorn() / barycenter() / wdl().
+ *
+ * Conventions (those of the reference's CUDA ABI,
+ * inst/include/cuda_interface.cuh:13-39):
+ *   - matrices are column-major (R layout), FP64, X(i,j) = X[i + j*rows];
+ *   - plain entry points take HOST pointers; inputs are borrowed read-only
+ *     for the call, outputs are caller-allocated and filled;
+ *   - every entry point returns an int status (RWIG_OK on success) where the
+ *     reference returned void; rwig_last_error() describes the last failure
+ *     on the calling thread.
+ * Entry points suffixed _dev take DEVICE pointers and a hipStream_t (passed
+ * as void*); they do not synchronise the stream.
+ *
+ * Results follow the reference's CPU path (src/sinkhorn_impl_*.cpp and
+ * src/barycenter_impl_*.cpp): solves run on the
+ * support of a and b, the loop runs while (iter < max_iter && err >= zero_tol),
+ * and return_status is 0 (err <= zero_tol), 1 (iter == max_iter) or 2.
+ */
+#ifndef RWIG_HIP_H
+#define RWIG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RWIG_OK 0
+#define RWIG_ERR_INVALID 1  /* bad shape or argument */
+#define RWIG_ERR_HIP 2      /* HIP runtime error */
+#define RWIG_ERR_NOMEM 3    /* device allocation failed or over budget */
+#define RWIG_ERR_NODEV 4    /* no usable gfx950 device */
+
+/* Replaces cuda_available_cpp() (src/check_cuda.cpp:6-12): 1 when a HIP
+ * device is usable by this library, else 0. */
+int rwig_available(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *rwig_last_error(void);
+
+/* Vanilla Sinkhorn. Replaces cuda_sinkhorn_vanilla
+ * (inst/include/cuda_interface.cuh:13-18, src/cuda_sinkhorn.cu:181-329),
+ * same argument order; semantics of Sinkhorn::compute_vanilla
+ * (src/sinkhorn_impl_vanilla.cpp:18-86). grad_a is read only when withgrad.
+ * return_status may be null. */
+int rwig_sinkhorn_vanilla(double *P, double *grad_a, double *u, double *v,
+                          double *loss, int *iter_out, double *err_out,
+                          const double *a, const double *b, const double *C,
+                          int m, int n, double reg, int withgrad, int max_iter,
+                          double zero_tol, int *return_status);
+
+/* Log-stabilised Sinkhorn: the GPU form of sinkhorn_log_cpp
+ * (src/sinkhorn.cpp:142-170, Sinkhorn::compute_log,
+ * src/sinkhorn_impl_log.cpp:225-294), which the reference runs on the CPU
+ * only. f, g are -inf off the support, as in the reference. */
+int rwig_sinkhorn_log(double *P, double *grad_a, double *f, double *g,
+                      double *loss, int *iter_out, double *err_out,
+                      const double *a, const double *b, const double *C,
+                      int m, int n, double reg, int withgrad, int max_iter,
+                      double zero_tol, int *return_status);
+
+/* Parallel Wasserstein barycenter. Replaces cuda_barycenter_parallel
+ * (inst/include/cuda_interface.cuh:20-26, src/cuda_barycenter.cu:268-404),
+ * same argument order; semantics of Barycenter::compute_parallel
+ * (src/barycenter_impl_parallel.cpp:19-53). b_ext is read only when
+ * withgrad; grad_A, grad_w, loss are written only when withgrad. */
+int rwig_barycenter_parallel(double *U, double *V, double *b, double *grad_A,
+                             double *grad_w, double *loss, int *iter_out,
+                             double *err_out, const double *A, const double *w,
+                             const double *C, const double *b_ext, int M,
+                             int N, int S, double reg, int withgrad,
+                             int max_iter, double zero_tol,
+                             int *return_status);
+
+/* Log-stabilised barycenter: the GPU form of barycenter_log_cpp
+ * (src/barycenter.cpp:168-207, Barycenter::compute_log,
+ * src/barycenter_impl_log.cpp:23-48), CPU-only in the reference. F, G take
+ * the place of U, V. The stopping error is the spectral norm of
+ * -KV/reg - log A, as in src/barycenter_impl_log.cpp:252. */
+int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A,
+                        double *grad_w, double *loss, int *iter_out,
+                        double *err_out, const double *A, const double *w,
+                        const double *C, const double *b_ext, int M, int N,
+                        int S, double reg, int withgrad, int max_iter,
+                        double zero_tol, int *return_status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RWIG_HIP_H */

@@ -1,0 +1,74 @@
+"""ctypes binding of librwig_hip.so (declarations in include/rwig_hip.h).
+
+The library is the product: there is no CPU fallback. Loading fails loudly
+when the shared object has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librwig_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "rwig_hip.h")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_d = ctypes.c_double
+_i = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# argument types of every exported entry point
+SIGNATURES = {
+    "rwig_available": ([], _i),
+    "rwig_last_error": ([], ctypes.c_char_p),
+    "rwig_sinkhorn_vanilla": ([_dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _i, _i, _d,
+                               _i, _i, _d, _ip], _i),
+    "rwig_sinkhorn_log": ([_dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _i, _i, _d, _i,
+                           _i, _d, _ip], _i),
+    "rwig_barycenter_parallel": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
+                                  _i, _i, _i, _d, _i, _i, _d, _ip], _i),
+    "rwig_barycenter_log": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _i,
+                             _i, _i, _d, _i, _i, _d, _ip], _i),
+}
+
+_lib = None
+
+
+class RwigError(RuntimeError):
+    """A non-zero status from librwig_hip.so."""
+
+    def __init__(self, status, message):
+        super().__init__(f"librwig_hip status {status}: {message}")
+        self.status = status
+
+
+def header_symbols():
+    """Names of the functions include/rwig_hip.h declares."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(rwig_\w+)\s*\(",
+                                 text, flags=re.M)))
+
+
+def lib():
+    """Load librwig_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m rwig_amd.build`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = handle
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        msg = lib().rwig_last_error()
+        raise RwigError(status, msg.decode() if msg else "")

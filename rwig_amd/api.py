@@ -1,0 +1,163 @@
+"""Host-side mirror of rwig's R entry points sinkhorn() and barycenter().
+
+Same control lists, defaults, "auto" method rule, argument checks and output
+element names as the reference (R/sinkhorn.R:85-177, R/barycenter.R:90-218,
+R/check_args.R:108-271, src/sinkhorn.cpp:28-43, :154-169,
+src/barycenter.cpp:43-59, :190-206). Every solve runs in librwig_hip.so on the
+GPU; ``use_cuda=False`` (the reference's CPU path) is not offered.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from ._lib import check, lib
+
+_SINKHORN_ARGS = ("reg", "with_grad", "use_cuda", "n_threads", "method", "threshold",
+                  "max_iter", "zero_tol", "verbose")
+
+
+def _fill_defaults(ctrl, kind):
+    """check_sinkhorn_args / check_barycenter_args (R/check_args.R:108-271)."""
+    ctrl = dict(ctrl or {})
+    bad = [k for k in ctrl if k not in _SINKHORN_ARGS]
+    if bad:
+        raise ValueError(f"{', '.join(bad)} not matching one of the {kind} arguments: "
+                         + ", ".join(f'"{a}"' for a in _SINKHORN_ARGS))
+    if kind == "barycenter":
+        ctrl.setdefault("reg", 0.1)
+    ctrl.setdefault("with_grad", False)
+    ctrl.setdefault("use_cuda", True)
+    ctrl.setdefault("n_threads", 0)
+    ctrl.setdefault("method", "auto")
+    ctrl.setdefault("threshold", 0.1)
+    ctrl.setdefault("max_iter", 1000)
+    ctrl.setdefault("zero_tol", 1e-6)
+    ctrl.setdefault("verbose", 10)
+    allowed = ("auto", "vanilla", "log") if kind == "sinkhorn" else ("auto", "parallel", "log")
+    if ctrl["method"] not in allowed:
+        raise ValueError("method must be from: " + ", ".join(f'"{m}"' for m in allowed))
+    if not ctrl["use_cuda"]:
+        raise ValueError("this build runs every solve on the GPU; use_cuda=False is not offered")
+    reg = ctrl.get("reg")
+    if reg is None or not np.isscalar(reg):
+        raise ValueError("reg must be a single value")
+    return ctrl
+
+
+def _auto_method(C, reg, threshold, small):
+    """R/sinkhorn.R:122-142: log iff min(exp(-min C/reg), exp(-max C/reg)) < threshold."""
+    k1 = math.exp(-float(np.min(C)) / reg)
+    k2 = math.exp(-float(np.max(C)) / reg)
+    return "log" if min(k1, k2) < threshold else small
+
+
+def _f64(x):
+    return np.asfortranarray(np.asarray(x, dtype=np.float64))
+
+
+def _p(x):
+    return None if x is None else x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def sinkhorn(a, b, C, sinkhorn_control=None):
+    """Entropic OT between histograms a (M) and b (N) under cost C (M x N).
+
+    Mirrors R/sinkhorn.R:85-177. Returns a dict with P, [grad_a], u|f, v|g,
+    loss, iter, err, return_status, method.
+    """
+    a = _f64(a).ravel()
+    b = _f64(b).ravel()
+    C = _f64(C)
+    if C.ndim != 2:
+        raise ValueError("C must be a numeric matrix!")
+    if a.size != C.shape[0] or b.size != C.shape[1]:
+        raise ValueError("a, b, C don't match in dimension!")
+    ctrl = _fill_defaults(sinkhorn_control, "sinkhorn")
+    reg = float(ctrl["reg"])
+    method = ctrl["method"]
+    if method == "auto":
+        method = _auto_method(C, reg, ctrl["threshold"], "vanilla")
+    M, N = C.shape
+    wg = bool(ctrl["with_grad"])
+    P = np.zeros((M, N), order="F")
+    grad = np.zeros(M)
+    u = np.zeros(M)
+    v = np.zeros(N)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    fn = lib().rwig_sinkhorn_vanilla if method == "vanilla" else lib().rwig_sinkhorn_log
+    check(fn(_p(P), _p(grad), _p(u), _p(v), ctypes.byref(loss), ctypes.byref(it),
+             ctypes.byref(err), _p(a), _p(b), _p(C), M, N, reg, int(wg),
+             int(ctrl["max_iter"]), float(ctrl["zero_tol"]), ctypes.byref(rc)))
+    out = {"P": P}
+    if wg:
+        out["grad_a"] = grad
+    names = ("u", "v") if method == "vanilla" else ("f", "g")
+    out[names[0]] = u
+    out[names[1]] = v
+    out.update(loss=loss.value, iter=it.value, err=err.value, return_status=rc.value,
+               method=method)
+    return out
+
+
+def barycenter(A, C, w, b_ext=None, barycenter_control=None):
+    """Wasserstein barycenter of the columns of A (M x S) with weights w.
+
+    Mirrors R/barycenter.R:90-218. Returns a dict with b, [grad_A, grad_w,
+    loss], U|F, V|G, iter, err, return_status, method.
+    """
+    A = _f64(A)
+    C = _f64(C)
+    w = _f64(w).ravel()
+    if A.ndim != 2:
+        raise ValueError("A must be a numeric matrix!")
+    if C.ndim != 2:
+        raise ValueError("C must be a numeric matrix!")
+    if A.shape[0] != C.shape[0]:
+        raise ValueError("number of rows of A and C must equal!")
+    ctrl = _fill_defaults(barycenter_control, "barycenter")
+    reg = float(ctrl["reg"])
+    wg = bool(ctrl["with_grad"])
+    if wg and b_ext is None:
+        raise ValueError("you must supply the `b_ext` with `with_grad = TRUE`!")
+    method = ctrl["method"]
+    if method == "auto":
+        method = _auto_method(C, reg, ctrl["threshold"], "parallel")
+    M, S = A.shape
+    N = C.shape[1]
+    if w.size != S:
+        raise ValueError("length of w must equal the number of columns of A")
+    bext = _f64(b_ext).ravel() if wg else None
+    if wg and bext.size != N:
+        raise ValueError("b_ext must have one entry per column of C")
+    b = np.zeros(N)
+    U = np.zeros((M, S), order="F")
+    V = np.zeros((N, S), order="F")
+    gA = np.zeros((M, S), order="F")
+    gw = np.zeros(S)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    fn = lib().rwig_barycenter_parallel if method == "parallel" else lib().rwig_barycenter_log
+    check(fn(_p(U), _p(V), _p(b), _p(gA), _p(gw), ctypes.byref(loss), ctypes.byref(it),
+             ctypes.byref(err), _p(A), _p(w), _p(C), _p(bext), M, N, S, reg, int(wg),
+             int(ctrl["max_iter"]), float(ctrl["zero_tol"]), ctypes.byref(rc)))
+    out = {"b": b}
+    if wg:
+        out.update(grad_A=gA, grad_w=gw, loss=loss.value)
+    names = ("U", "V") if method == "parallel" else ("F", "G")
+    out[names[0]] = U
+    out[names[1]] = V
+    out.update(iter=it.value, err=err.value, return_status=rc.value, method=method)
+    return out
+
+
+def cuda_available():
+    """cuda_available_cpp() (src/check_cuda.cpp:6-12): is a HIP device usable."""
+    return bool(lib().rwig_available())

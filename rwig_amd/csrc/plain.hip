@@ -1,0 +1,981 @@
+// Plain FP64 GPU form of rwig's four solvers and their unrolled adjoints:
+// the drop-in for sinkhorn_vanilla_cpp / sinkhorn_log_cpp /
+// barycenter_parallel_cpp / barycenter_log_cpp (src/sinkhorn.cpp,
+// src/barycenter.cpp), with the semantics of the reference's CPU core
+// (src/sinkhorn_impl_vanilla.cpp, src/sinkhorn_impl_log.cpp,
+// src/barycenter_impl_parallel.cpp, src/barycenter_impl_log.cpp).
+//
+// Each iteration is a short chain of streaming kernels over the cost or
+// Gibbs matrix; the loop condition lives on the device (LoopState) and the
+// host polls it once per batch of enqueued iterations. The forward pass
+// records, beside the iterates, the products the adjoint needs (K v_l,
+// K^T u_l, the soft-min vectors of every iteration), so each backward step is
+// two passes over the matrix instead of the reference's four or five.
+#include <algorithm>
+
+#include "kernels_fp64.hpp"
+
+namespace rwig {
+
+namespace {
+
+void init_loop(hipStream_t st, LoopState *dls, int maxiter, double tol) {
+  LoopState h{};
+  h.iter = 0;
+  h.err = 1000.0;  // inst/include/sinkhorn_impl.hpp:85-88
+  h.maxiter = maxiter;
+  h.tol = tol;
+  h.active = (h.iter < maxiter && h.err >= tol) ? 1 : 0;
+  h2d(dls, &h, 1, st);
+}
+
+// Enqueue iterations in growing batches until the device loop reports done.
+template <class Iter>
+LoopState run_loop(hipStream_t st, LoopState *dls, int maxiter, Iter &&iteration) {
+  LoopState hs{};
+  d2h(&hs, dls, 1, st);
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  int batch = 4;
+  while (hs.active) {
+    const int n = std::min(batch, maxiter - hs.iter);
+    for (int k = 0; k < n; ++k) iteration();
+    d2h(&hs, dls, 1, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    batch = std::min(batch * 2, 64);
+  }
+  return hs;
+}
+
+// Device-side end of one iteration: iter++, err, and the loop condition
+// while (iter < maxiter && err >= tol) of the reference.
+__device__ __forceinline__ void finish_iteration(LoopState *ls, double err) {
+  ls->err = err;
+  ls->iter = ls->iter + 1;
+  ls->active = (ls->iter < ls->maxiter && err >= ls->tol) ? 1 : 0;
+}
+
+template <class T> void fill(hipStream_t st, T *p, long n, T v) {
+  launch_map(st, n, [=] __device__(long k) { p[k] = v; });
+}
+
+// Jacobi eigenvalue sweeps on the s x s symmetric matrix G (in place); returns
+// the largest eigenvalue. Same rotation sequence as the oracle's
+// spectral_norm().
+__device__ double jacobi_lmax(double *G, int s) {
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < s; ++p)
+      for (int q = p + 1; q < s; ++q) off += G[p + q * s] * G[p + q * s];
+    if (off == 0.0) break;
+    for (int p = 0; p < s; ++p)
+      for (int q = p + 1; q < s; ++q) {
+        const double apq = G[p + q * s];
+        if (apq == 0.0) continue;
+        const double app = G[p + p * s], aqq = G[q + q * s];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < s; ++k) {
+          const double gkp = G[k + p * s], gkq = G[k + q * s];
+          G[k + p * s] = c * gkp - sn * gkq;
+          G[k + q * s] = sn * gkp + c * gkq;
+        }
+        for (int k = 0; k < s; ++k) {
+          const double gpk = G[p + k * s], gqk = G[q + k * s];
+          G[p + k * s] = c * gpk - sn * gqk;
+          G[q + k * s] = sn * gpk + c * gqk;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int p = 0; p < s; ++p) lmax = fmax(lmax, G[p + p * s]);
+  return lmax;
+}
+
+// Spectral norm of the M x S matrix e(i, s) (Armadillo norm(X, 2),
+// src/barycenter_impl_log.cpp:252): Gram matrix by block sums, then Jacobi
+// on thread 0. fin(norm) runs on thread 0.
+template <class E, class Fin>
+__global__ void __launch_bounds__(1024)
+k_spectral(long M, int S, E e, Fin fin, const LoopState *gate) {
+  __shared__ double G[32 * 32];
+  __shared__ double sh[1024 / kWave];
+  if (gate && !gate->active) return;
+  for (int p = 0; p < S; ++p)
+    for (int q = p; q < S; ++q) {
+      double acc = 0.0;
+      for (long i = threadIdx.x; i < M; i += blockDim.x) acc += e(i, p) * e(i, q);
+      acc = block_sum(acc, sh);
+      if (threadIdx.x == 0) {
+        G[p + q * S] = acc;
+        G[q + p * S] = acc;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) fin(sqrt(jacobi_lmax(G, S)));
+}
+
+template <class E, class Fin>
+void spectral(hipStream_t st, long M, int S, E e, Fin fin, const LoopState *gate) {
+  hipLaunchKernelGGL((k_spectral<E, Fin>), dim3(1), dim3(1024), 0, st, M, S, e, fin, gate);
+  check_launch();
+}
+
+// out(s) = scale * sum_j acc(j, s), one block per s.
+__global__ void __launch_bounds__(kTB)
+k_colsum(const double *acc, int N, double scale, double *out) {
+  __shared__ double sh[kTB / kWave];
+  const int s = blockIdx.x;
+  double a = 0.0;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) a += acc[j + (long)s * N];
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) out[s] = scale * a;
+}
+
+// Partial sums of the regularised transport cost
+// sum C .* P + reg * sum P .* (log P - 1)  (src/sinkhorn_impl_vanilla.cpp:69)
+__global__ void __launch_bounds__(kTB)
+k_loss_part(long n, const double *__restrict__ C, const double *__restrict__ P,
+            double *part) {
+  __shared__ double sh[kTB / kWave];
+  double a = 0.0, b = 0.0;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    const double p = P[k];
+    a += C[k] * p;
+    b += p * (log(p) - 1.0);
+  }
+  a = block_sum(a, sh);
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[kSumBlocks + blockIdx.x] = b;
+  }
+}
+
+void sinkhorn_loss(hipStream_t st, long n, const double *C, const double *P, double reg,
+                   double *part /* 2*kSumBlocks */, double *loss) {
+  hipLaunchKernelGGL(k_loss_part, dim3(kSumBlocks), dim3(kTB), 0, st, n, C, P, part);
+  check_launch();
+  reduce2(
+      st, (long)kSumBlocks, [=] __device__(long k) { return part[k]; }, (long)kSumBlocks,
+      [=] __device__(long k) { return part[kSumBlocks + k]; },
+      [=] __device__(double a, double b) { *loss = a + reg * b; });
+}
+
+// ---------------------------------------------------------------- support reduction
+
+struct Reduced {
+  std::vector<int> ai, bi;
+  std::vector<double> a, b, C;
+  int M = 0, N = 0;
+};
+
+// find(a != 0), find(b != 0), C.submat(a_ind, b_ind)
+// (src/sinkhorn_impl_vanilla.cpp:29-35)
+Reduced reduce_support(const double *a, const double *b, const double *C, int m, int n) {
+  Reduced r;
+  for (int i = 0; i < m; ++i)
+    if (a[i] != 0.0) r.ai.push_back(i);
+  for (int j = 0; j < n; ++j)
+    if (b[j] != 0.0) r.bi.push_back(j);
+  r.M = (int)r.ai.size();
+  r.N = (int)r.bi.size();
+  r.a.resize(r.M);
+  r.b.resize(r.N);
+  for (int i = 0; i < r.M; ++i) r.a[i] = a[r.ai[i]];
+  for (int j = 0; j < r.N; ++j) r.b[j] = b[r.bi[j]];
+  r.C.resize((size_t)r.M * r.N);
+  for (int j = 0; j < r.N; ++j)
+    for (int i = 0; i < r.M; ++i) r.C[i + (size_t)j * r.M] = C[r.ai[i] + (size_t)r.bi[j] * m];
+  return r;
+}
+
+int return_status(double err, double tol, int iter, int maxiter) {
+  if (err <= tol) return 0;  // src/sinkhorn.cpp:88-95 with the CPU's == test
+  if (iter == maxiter) return 1;
+  return 2;
+}
+
+}  // namespace
+
+// ============================================================================
+// Vanilla Sinkhorn (src/sinkhorn_impl_vanilla.cpp:18-214)
+// ============================================================================
+
+struct SinkhornResult {
+  int iter = 0;
+  double err = 0.0;
+};
+
+// All pointers are device pointers of the reduced (support) problem.
+// u (M), v (N), P (M*N), loss (1), grad (M, withgrad only) are outputs.
+SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const double *b,
+                                    const double *C, int M, int N, double reg, bool wg,
+                                    int maxiter, double tol, double *u, double *v, double *P,
+                                    double *loss, double *grad) {
+  const long MN = (long)M * N;
+  DBuf<double> Kb(MN), Kvb(M), KTub(N), partb(gemv_n_part_size(M, N, 1));
+  DBuf<double> sump(2 * kSumBlocks);
+  DBuf<LoopState> lsb(1);
+  double *K = Kb.get(), *Kv = Kvb.get(), *KTu = KTub.get(), *part = partb.get();
+  LoopState *ls = lsb.get();
+  const int L = maxiter;
+  DBuf<double> uhb, vhb, yhb, zhb;
+  if (wg) {
+    uhb.reset((size_t)(L + 1) * M);
+    vhb.reset((size_t)(L + 1) * N);
+    yhb.reset((size_t)(L + 1) * M);
+    zhb.reset((size_t)(L + 1) * N);
+  }
+  double *uh = uhb.get(), *vh = vhb.get(), *yh = yhb.get(), *zh = zhb.get();
+
+  launch_map(st, MN, [=] __device__(long k) { K[k] = exp(-C[k] / reg); });
+  fill(st, u, M, 1.0);
+  fill(st, v, N, 1.0);
+  if (wg) {
+    fill(st, uh, M, 1.0);
+    fill(st, vh, N, 1.0);
+  }
+  init_loop(st, ls, maxiter, tol);
+  // Kv = K v  (v = 1)
+  gemv_n(st, K, M, N, M, v, N, 1, part,
+         [=] __device__(long k, double s) {
+           Kv[k] = s;
+           if (wg) yh[k] = s;
+         },
+         nullptr);
+
+  auto iteration = [&]() {
+    // u = a / Kv
+    launch_map(
+        st, M,
+        [=] __device__(long i) {
+          const double ui = a[i] / Kv[i];
+          u[i] = ui;
+          if (wg) uh[(long)(ls->iter + 1) * M + i] = ui;
+        },
+        ls);
+    // KTu = K^T u, v = b / KTu
+    gemv_t(
+        st, K, M, N, M, u, M, 1,
+        [=] __device__(int j, const double *acc) {
+          const double z = acc[0];
+          KTu[j] = z;
+          const double vj = b[j] / z;
+          v[j] = vj;
+          if (wg) {
+            const long l = ls->iter + 1;
+            vh[l * N + j] = vj;
+            zh[l * N + j] = z;
+          }
+        },
+        ls);
+    // Kv = K v
+    gemv_n(
+        st, K, M, N, M, v, N, 1, part,
+        [=] __device__(long i, double s) {
+          Kv[i] = s;
+          if (wg) yh[(long)(ls->iter + 1) * M + i] = s;
+        },
+        ls);
+    // err = ||u .* Kv - a|| + ||v .* KTu - b||
+    reduce2(
+        st, (long)M,
+        [=] __device__(long i) {
+          const double r = u[i] * Kv[i] - a[i];
+          return r * r;
+        },
+        (long)N,
+        [=] __device__(long j) {
+          const double r = v[j] * KTu[j] - b[j];
+          return r * r;
+        },
+        [=] __device__(double s1, double s2) { finish_iteration(ls, sqrt(s1) + sqrt(s2)); }, ls);
+  };
+  const LoopState fin = run_loop(st, ls, maxiter, iteration);
+
+  // P = diag(u) K diag(v)
+  launch_map(st, MN, [=] __device__(long k) {
+    const long i = k % M, j = k / M;
+    P[k] = (u[i] * K[k]) * v[j];
+  });
+  sinkhorn_loss(st, MN, C, P, reg, sump.get(), loss);
+
+  if (wg) {
+    // adjoint: src/sinkhorn_impl_vanilla.cpp:169-214
+    const int Lf = fin.iter;
+    DBuf<double> PbKb(MN), ubarb(M), xb(M), yb(N);
+    double *PbK = PbKb.get(), *ubar = ubarb.get(), *x = xb.get(), *y = yb.get();
+    launch_map(st, MN, [=] __device__(long k) { PbK[k] = (C[k] + reg * log(P[k])) * K[k]; });
+    fill(st, grad, M, 0.0);
+    for (int l = Lf; l > 0; --l) {
+      const double *ylm1 = yh + (long)(l - 1) * M;  // K v_{l-1}
+      const double *zl = zh + (long)l * N;        // K^T u_l
+      const double *ul = uh + (long)l * M;
+      const double *vl = vh + (long)l * N;
+      if (l == Lf) {
+        // vbar = PbarK^T u; y = (vbar .* v) / (K^T u)
+        gemv_t(
+            st, PbK, M, N, M, u, M, 1,
+            [=] __device__(int j, const double *acc) { y[j] = (acc[0] * v[j]) / zl[j]; },
+            nullptr);
+        // ubar = PbarK v - K y
+        gemv_n(st, PbK, M, N, M, v, N, 1, part,
+               [=] __device__(long i, double s) { ubar[i] = s; }, nullptr);
+        gemv_n(
+            st, K, M, N, M, y, N, 1, part,
+            [=] __device__(long i, double s) {
+              const double ub = ubar[i] - s;
+              ubar[i] = ub;
+              grad[i] += ub / ylm1[i];
+              x[i] = (ub * ul[i]) / ylm1[i];
+            },
+            nullptr);
+      } else {
+        // vbar = -K^T x; y = (vbar .* v_l) / (K^T u_l)
+        gemv_t(
+            st, K, M, N, M, x, M, 1,
+            [=] __device__(int j, const double *acc) {
+              const double vb = -acc[0];
+              y[j] = (vb * vl[j]) / zl[j];
+            },
+            nullptr);
+        // ubar = -K y; grad += ubar / (K v_{l-1}); x for step l-1
+        gemv_n(
+            st, K, M, N, M, y, N, 1, part,
+            [=] __device__(long i, double s) {
+              const double ub = -s;
+              ubar[i] = ub;
+              grad[i] += ub / ylm1[i];
+              x[i] = (ub * ul[i]) / ylm1[i];
+            },
+            nullptr);
+      }
+    }
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  return {fin.iter, fin.err};
+}
+
+// ============================================================================
+// Log-stabilised Sinkhorn (src/sinkhorn_impl_log.cpp:225-520)
+// ============================================================================
+
+SinkhornResult sinkhorn_log_dev(hipStream_t st, const double *a, const double *b,
+                                const double *C, int M, int N, double reg, bool wg,
+                                int maxiter, double tol, double *f, double *g, double *P,
+                                double *loss, double *grad) {
+  const long MN = (long)M * N;
+  DBuf<double> logab(M), logbb(N), rminb(M), cminb(N), cmin2b(N);
+  DBuf<double> pmb(lse_row_part_size(M, N, 1)), psb(lse_row_part_size(M, N, 1));
+  DBuf<double> sump(2 * kSumBlocks);
+  DBuf<LoopState> lsb(1);
+  double *loga = logab.get(), *logb = logbb.get(), *rmin = rminb.get(), *cmin = cminb.get(),
+         *cmin2 = cmin2b.get(), *pm = pmb.get(), *ps = psb.get();
+  LoopState *ls = lsb.get();
+  const int L = maxiter;
+  DBuf<double> fhb, ghb, rmhb, cmhb;
+  if (wg) {
+    fhb.reset((size_t)(L + 1) * M);
+    ghb.reset((size_t)(L + 1) * N);
+    rmhb.reset((size_t)(L + 1) * M);
+    cmhb.reset((size_t)(L + 1) * N);
+  }
+  double *fh = fhb.get(), *gh = ghb.get(), *rmh = rmhb.get(), *cmh = cmhb.get();
+
+  fill(st, f, M, 0.0);
+  fill(st, g, N, 0.0);
+  if (wg) {
+    fill(st, fh, M, 0.0);
+    fill(st, gh, N, 0.0);
+  }
+  launch_map(st, M, [=] __device__(long i) { loga[i] = log(a[i]); });
+  launch_map(st, N, [=] __device__(long j) { logb[j] = log(b[j]); });
+  init_loop(st, ls, maxiter, tol);
+  // Rminrow of R(0, 0)  (src/sinkhorn_impl_log.cpp:325)
+  lse_row(st, C, M, N, M, f, M, g, N, 1, reg, pm, ps,
+          [=] __device__(long i, double r) { rmin[i] = r; }, nullptr);
+
+  auto iteration = [&]() {
+    // f += reg * log a + Rminrow
+    launch_map(
+        st, M,
+        [=] __device__(long i) {
+          const double fi = f[i] + (reg * loga[i] + rmin[i]);
+          f[i] = fi;
+          if (wg) fh[(long)(ls->iter + 1) * M + i] = fi;
+        },
+        ls);
+    // g += reg * log b + Rmincol(f, g)
+    lse_col(
+        st, C, M, N, M, f, M, g, N, 1, reg,
+        [=] __device__(int j, const double *r) {
+          cmin[j] = r[0];
+          const double gj = g[j] + (reg * logb[j] + r[0]);
+          g[j] = gj;
+          if (wg) gh[(long)(ls->iter + 1) * N + j] = gj;
+        },
+        ls);
+    // Rminrow, Rmincol of R(f, g) for the error (and the next f update)
+    lse_row(
+        st, C, M, N, M, f, M, g, N, 1, reg, pm, ps,
+        [=] __device__(long i, double r) {
+          rmin[i] = r;
+          if (wg) rmh[(long)(ls->iter + 1) * M + i] = r;
+        },
+        ls);
+    lse_col(
+        st, C, M, N, M, f, M, g, N, 1, reg,
+        [=] __device__(int j, const double *r) {
+          cmin2[j] = r[0];
+          if (wg) cmh[(long)(ls->iter + 1) * N + j] = r[0];
+        },
+        ls);
+    // err = ||-Rminrow/reg - log a|| + ||-Rmincol/reg - log b||
+    reduce2(
+        st, (long)M,
+        [=] __device__(long i) {
+          const double r = -rmin[i] / reg - loga[i];
+          return r * r;
+        },
+        (long)N,
+        [=] __device__(long j) {
+          const double r = -cmin2[j] / reg - logb[j];
+          return r * r;
+        },
+        [=] __device__(double s1, double s2) { finish_iteration(ls, sqrt(s1) + sqrt(s2)); }, ls);
+  };
+  const LoopState fin = run_loop(st, ls, maxiter, iteration);
+
+  // P = exp(-R(f, g) / reg)
+  launch_map(st, MN, [=] __device__(long k) {
+    const long i = k % M, j = k / M;
+    const double r = (C[k] - f[i]) - g[j];
+    P[k] = exp(-r / reg);
+  });
+  sinkhorn_loss(st, MN, C, P, reg, sump.get(), loss);
+
+  if (wg) {
+    const int Lf = fin.iter;
+    DBuf<double> PbPb(MN), fbarb(M), gbarb(N), rsb(M), onesb(M > N ? M : N),
+        partb(gemv_n_part_size(M, N, 1));
+    double *PbP = PbPb.get(), *fbar = fbarb.get(), *gbar = gbarb.get(), *rs = rsb.get(),
+           *ones = onesb.get(), *part = partb.get();
+    launch_map(st, MN, [=] __device__(long k) { PbP[k] = (C[k] + reg * log(P[k])) * P[k]; });
+    fill(st, ones, (long)(M > N ? M : N), 1.0);
+    fill(st, grad, M, 0.0);
+    for (int l = Lf; l > 0; --l) {
+      const double *fl = fh + (long)l * M, *gl = gh + (long)l * N;
+      const double *rml = rmh + (long)l * M, *cml = cmh + (long)l * N;
+      if (l == Lf) {
+        // gbar = PbarP^T 1 / reg ; fbar = -W gbar + PbarP 1 / reg
+        gemv_t(st, PbP, M, N, M, ones, M, 1,
+               [=] __device__(int j, const double *acc) { gbar[j] = acc[0] / reg; }, nullptr);
+        gemv_n(st, PbP, M, N, M, ones, N, 1, part,
+               [=] __device__(long i, double s) { rs[i] = s / reg; }, nullptr);
+        wrow(
+            st, C, M, N, M, fl, M, gl, N, 1, reg, cml, gbar, part,
+            [=] __device__(long i, double s) {
+              const double fb = -s + rs[i];
+              fbar[i] = fb;
+              grad[i] += fb / a[i];
+            },
+            nullptr);
+      } else {
+        // gbar = -X fbar, X the row soft-max of -R(f_l, g_l)/reg
+        wcol(
+            st, C, M, N, M, fl, M, gl, N, 1, reg, rml, fbar,
+            [=] __device__(int j, const double *r) { gbar[j] = -r[0]; }, nullptr);
+        // fbar = -W gbar, W the column soft-max of -R(f_l, g_l)/reg
+        wrow(
+            st, C, M, N, M, fl, M, gl, N, 1, reg, cml, gbar, part,
+            [=] __device__(long i, double s) {
+              const double fb = -s;
+              fbar[i] = fb;
+              grad[i] += fb / a[i];
+            },
+            nullptr);
+      }
+    }
+    launch_map(st, M, [=] __device__(long i) { grad[i] *= reg; });
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  return {fin.iter, fin.err};
+}
+
+// ============================================================================
+// Parallel barycenter (src/barycenter_impl_parallel.cpp:19-207)
+// ============================================================================
+
+// Device pointers. A (M x S), C (M x N), w (S), bext (N, withgrad) inputs;
+// b (N, normalised on return), U (M x S), V (N x S), gA (M x S), gw (S),
+// loss (1) outputs.
+SinkhornResult barycenter_parallel_dev(hipStream_t st, const double *A, const double *C,
+                                       const double *w, const double *bext, int M, int N, int S,
+                                       double reg, bool wg, int maxiter, double tol, double *b,
+                                       double *U, double *V, double *gA, double *gw,
+                                       double *loss) {
+  const long MN = (long)M * N, MS = (long)M * S, NS = (long)N * S;
+  DBuf<double> Kb(MN), KVb(MS), KTUb(NS), partb(gemv_n_part_size(M, N, S));
+  DBuf<LoopState> lsb(1);
+  double *K = Kb.get(), *KV = KVb.get(), *KTU = KTUb.get(), *part = partb.get();
+  LoopState *ls = lsb.get();
+  const int L = maxiter;
+  DBuf<double> Uhb, Vhb, bhb, KVhb, KTUhb;
+  if (wg) {
+    Uhb.reset((size_t)(L + 1) * MS);
+    Vhb.reset((size_t)(L + 1) * NS);
+    bhb.reset((size_t)(L + 1) * N);
+    KVhb.reset((size_t)(L + 1) * MS);
+    KTUhb.reset((size_t)(L + 1) * NS);
+  }
+  double *Uh = Uhb.get(), *Vh = Vhb.get(), *bh = bhb.get(), *KVh = KVhb.get(),
+         *KTUh = KTUhb.get();
+
+  launch_map(st, MN, [=] __device__(long k) { K[k] = exp(-C[k] / reg); });
+  fill(st, b, N, 0.0);
+  fill(st, U, MS, 1.0);
+  fill(st, V, NS, 1.0);
+  if (wg) {
+    fill(st, Uh, MS, 1.0);
+    fill(st, Vh, NS, 1.0);
+    fill(st, bh, N, 0.0);
+  }
+  init_loop(st, ls, maxiter, tol);
+  gemv_n(st, K, M, N, M, V, N, S, part,
+         [=] __device__(long k, double s) {
+           KV[k] = s;
+           if (wg) KVh[k] = s;
+         },
+         nullptr);
+
+  auto iteration = [&]() {
+    // U = A / KV
+    launch_map(
+        st, MS,
+        [=] __device__(long k) {
+          const double uk = A[k] / KV[k];
+          U[k] = uk;
+          if (wg) Uh[(long)(ls->iter + 1) * MS + k] = uk;
+        },
+        ls);
+    // KTU = K^T U; b = prod_s KTU^w_s; V = b / KTU
+    gemv_t(
+        st, K, M, N, M, U, M, S,
+        [=] __device__(int j, const double *acc) {
+          double p = 1.0;
+          for (int s = 0; s < S; ++s) p *= pow(acc[s], w[s]);
+          b[j] = p;
+          const long l = ls->iter + 1;
+          if (wg) bh[l * N + j] = p;
+          for (int s = 0; s < S; ++s) {
+            const double vj = p / acc[s];
+            KTU[j + (long)s * N] = acc[s];
+            V[j + (long)s * N] = vj;
+            if (wg) {
+              Vh[l * NS + j + (long)s * N] = vj;
+              KTUh[l * NS + j + (long)s * N] = acc[s];
+            }
+          }
+        },
+        ls);
+    // KV = K V
+    gemv_n(
+        st, K, M, N, M, V, N, S, part,
+        [=] __device__(long k, double s) {
+          KV[k] = s;
+          if (wg) KVh[(long)(ls->iter + 1) * MS + k] = s;
+        },
+        ls);
+    // err = ||U .* KV - A||_F
+    reduce2(
+        st, MS,
+        [=] __device__(long k) {
+          const double r = U[k] * KV[k] - A[k];
+          return r * r;
+        },
+        0L, [=] __device__(long) { return 0.0; },
+        [=] __device__(double s1, double) { finish_iteration(ls, sqrt(s1)); }, ls);
+  };
+  const LoopState fin = run_loop(st, ls, maxiter, iteration);
+
+  if (wg) {
+    const int Lf = fin.iter;
+    DBuf<double> Xb(NS), Ymb(MS), gwab(NS);
+    double *X = Xb.get(), *Xm = Ymb.get(), *gwa = gwab.get();
+    fill(st, gA, MS, 0.0);
+    fill(st, gwa, NS, 0.0);
+    for (int l = Lf; l > 0; --l) {
+      const double *KTUl = KTUh + (long)l * NS, *Vl = Vh + (long)l * NS, *bl = bh + (long)l * N;
+      const double *Ul = Uh + (long)l * MS, *KVlm1 = KVh + (long)(l - 1) * MS;
+      if (l == Lf) {
+        // bbar = 2 (b_L - b_ext); X = (bbar w^T) .* V_L; grad_w += log(KTU)^T (bbar .* b_L)
+        launch_map(st, NS, [=] __device__(long k) {
+          const long j = k % N, s = k / N;
+          const double bb = 2.0 * (b[j] - bext[j]);
+          X[k] = (bb * w[s]) * Vl[k];
+          gwa[k] += log(KTUl[k]) * (bb * bl[j]);
+        });
+      } else {
+        // Vbar = -K^T Xm; bbar = rowsum(Vbar / KTU);
+        // X = (bbar w^T - Vbar / KTU) .* V_l; grad_w += log(KTU)^T (bbar .* b_l)
+        gemv_t(
+            st, K, M, N, M, Xm, M, S,
+            [=] __device__(int j, const double *acc) {
+              double bb = 0.0;
+              for (int s = 0; s < S; ++s) bb += (-acc[s]) / KTUl[j + (long)s * N];
+              for (int s = 0; s < S; ++s) {
+                const long k = j + (long)s * N;
+                X[k] = (bb * w[s] - (-acc[s]) / KTUl[k]) * Vl[k];
+                gwa[k] += log(KTUl[k]) * (bb * bl[j]);
+              }
+            },
+            nullptr);
+      }
+      // Ubar = K X; grad_A += Ubar / (K V_{l-1}); Xm = (Ubar .* U_l) / (K V_{l-1})
+      gemv_n(
+          st, K, M, N, M, X, N, S, part,
+          [=] __device__(long k, double s) {
+            gA[k] += s / KVlm1[k];
+            Xm[k] = (s * Ul[k]) / KVlm1[k];
+          },
+          nullptr);
+    }
+    hipLaunchKernelGGL(k_colsum, dim3(S), dim3(kTB), 0, st, gwa, N, 1.0, gw);
+    check_launch();
+  }
+  // b = b / sum(b); loss = ||b - b_ext||^2  (src/barycenter_impl_parallel.cpp:39-43)
+  DBuf<double> totb(1);
+  double *tot = totb.get();
+  reduce2(
+      st, (long)N, [=] __device__(long j) { return b[j]; }, 0L,
+      [=] __device__(long) { return 0.0; }, [=] __device__(double s, double) { *tot = s; });
+  launch_map(st, N, [=] __device__(long j) { b[j] = b[j] / *tot; });
+  if (wg) {
+    reduce2(
+        st, (long)N,
+        [=] __device__(long j) {
+          const double r = b[j] - bext[j];
+          return r * r;
+        },
+        0L, [=] __device__(long) { return 0.0; },
+        [=] __device__(double s, double) { *loss = s; });
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  return {fin.iter, fin.err};
+}
+
+// ============================================================================
+// Log barycenter (src/barycenter_impl_log.cpp:23-382)
+// ============================================================================
+
+SinkhornResult barycenter_log_dev(hipStream_t st, const double *A, const double *C,
+                                  const double *w, const double *bext, int M, int N, int S,
+                                  double reg, bool wg, int maxiter, double tol, double *b,
+                                  double *F, double *G, double *gA, double *gw,
+                                  double *loss) {
+  RWIG_REQUIRE(S <= 32, "log barycenter supports at most 32 measures");
+  const long MS = (long)M * S, NS = (long)N * S;
+  DBuf<double> logAb(MS), KVb(MS), KTUb(NS), logbb(N);
+  DBuf<double> pmb(lse_row_part_size(M, N, S)), psb(lse_row_part_size(M, N, S));
+  DBuf<LoopState> lsb(1);
+  double *logA = logAb.get(), *KV = KVb.get(), *KTU = KTUb.get(), *logb = logbb.get(),
+         *pm = pmb.get(), *ps = psb.get();
+  LoopState *ls = lsb.get();
+  const int L = maxiter;
+  DBuf<double> Fhb, Ghb, KVhb, KTUhb;
+  if (wg) {
+    Fhb.reset((size_t)(L + 1) * MS);
+    Ghb.reset((size_t)(L + 1) * NS);
+    KVhb.reset((size_t)(L + 1) * MS);
+    KTUhb.reset((size_t)(L + 1) * NS);
+  }
+  double *Fh = Fhb.get(), *Gh = Ghb.get(), *KVh = KVhb.get(), *KTUh = KTUhb.get();
+
+  fill(st, logb, N, 0.0);
+  fill(st, F, MS, 0.0);
+  fill(st, G, NS, 0.0);
+  if (wg) {
+    fill(st, Fh, MS, 0.0);
+    fill(st, Gh, NS, 0.0);
+  }
+  launch_map(st, MS, [=] __device__(long k) { logA[k] = log(A[k]); });
+  init_loop(st, ls, maxiter, tol);
+  lse_row(st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
+          [=] __device__(long k, double r) { KV[k] = r; }, nullptr);
+
+  auto iteration = [&]() {
+    // F += reg * log A + KV
+    launch_map(
+        st, MS,
+        [=] __device__(long k) {
+          const double fk = F[k] + (reg * logA[k] + KV[k]);
+          F[k] = fk;
+          if (wg) Fh[(long)(ls->iter + 1) * MS + k] = fk;
+        },
+        ls);
+    // KTU = mincol(R_s(F, G)); logb = -(G + KTU) w / reg; G += reg logb 1^T + KTU
+    lse_col(
+        st, C, M, N, M, F, M, G, N, S, reg,
+        [=] __device__(int j, const double *r) {
+          double acc = 0.0;
+          for (int s = 0; s < S; ++s) acc += -(G[j + (long)s * N] + r[s]) * w[s];
+          const double lb = acc / reg;
+          logb[j] = lb;
+          const long l = ls->iter + 1;
+          for (int s = 0; s < S; ++s) {
+            const long k = j + (long)s * N;
+            KTU[k] = r[s];
+            const double gk = G[k] + (reg * lb + r[s]);
+            G[k] = gk;
+            if (wg) {
+              Gh[l * NS + k] = gk;
+              KTUh[l * NS + k] = r[s];
+            }
+          }
+        },
+        ls);
+    // KV = minrow(R_s(F, G))
+    lse_row(
+        st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
+        [=] __device__(long k, double r) {
+          KV[k] = r;
+          if (wg) KVh[(long)(ls->iter + 1) * MS + k] = r;
+        },
+        ls);
+    // err = norm(-KV/reg - log A, 2): spectral norm
+    spectral(
+        st, (long)M, S,
+        [=] __device__(long i, int s) {
+          const long k = i + (long)s * M;
+          return -KV[k] / reg - logA[k];
+        },
+        [=] __device__(double nrm) { finish_iteration(ls, nrm); }, ls);
+  };
+  const LoopState fin = run_loop(st, ls, maxiter, iteration);
+
+  // b = exp(logb) / sum  (src/barycenter_impl_log.cpp:268-271)
+  DBuf<double> totb(1);
+  double *tot = totb.get();
+  launch_map(st, N, [=] __device__(long j) { b[j] = exp(logb[j]); });
+  reduce2(
+      st, (long)N, [=] __device__(long j) { return b[j]; }, 0L,
+      [=] __device__(long) { return 0.0; }, [=] __device__(double s, double) { *tot = s; });
+  launch_map(st, N, [=] __device__(long j) { b[j] = b[j] / *tot; });
+  if (wg) {
+    reduce2(
+        st, (long)N,
+        [=] __device__(long j) {
+          const double r = b[j] - bext[j];
+          return r * r;
+        },
+        0L, [=] __device__(long) { return 0.0; },
+        [=] __device__(double s, double) { *loss = s; });
+
+    const int Lf = fin.iter;
+    DBuf<double> Fbarb(MS), Gbarb(NS), lbbb(N), Yb(NS), gwab(NS),
+        partb(gemv_n_part_size(M, N, S));
+    double *Fbar = Fbarb.get(), *Gbar = Gbarb.get(), *lbb = lbbb.get(), *Y = Yb.get(),
+           *gwa = gwab.get(), *part = partb.get();
+    fill(st, gA, MS, 0.0);
+    fill(st, gwa, NS, 0.0);
+    fill(st, Fbar, MS, 0.0);
+    fill(st, Gbar, NS, 0.0);
+    for (int l = Lf; l > 0; --l) {
+      const double *Fl = Fh + (long)l * MS, *Gl = Gh + (long)l * NS;
+      const double *Glm1 = Gh + (long)(l - 1) * NS;
+      const double *KVl = KVh + (long)l * MS, *KTUl = KTUh + (long)l * NS;
+      if (l != Lf) {
+        // Gbar(:,s) = -X_s^T Fbar(:,s); logbbar = reg * rowsum(Gbar)
+        wcol(
+            st, C, M, N, M, Fl, M, Gl, N, S, reg, KVl, Fbar,
+            [=] __device__(int j, const double *r) {
+              double acc = 0.0;
+              for (int s = 0; s < S; ++s) {
+                Gbar[j + (long)s * N] = -r[s];
+                acc += -r[s];
+              }
+              lbb[j] = reg * acc;
+            },
+            nullptr);
+      } else {
+        // logbbar = 2 (b - b_ext) .* b with the normalised b
+        launch_map(st, N, [=] __device__(long j) { lbb[j] = 2.0 * (b[j] - bext[j]) * b[j]; });
+      }
+      // Fbar(:,s) = W_s ((w_s / reg) logbbar - Gbar(:,s)); W_s the column
+      // soft-max of -R_s(F_l, G_{l-1}) / reg
+      const bool last = (l == Lf);
+      launch_map(st, NS, [=] __device__(long k) {
+        const long j = k % N, s = k / N;
+        const double t = (w[s] / reg) * lbb[j];
+        Y[k] = last ? t : t - Gbar[k];
+        gwa[k] += (Glm1[k] + KTUl[k]) * lbb[j];
+      });
+      wrow(
+          st, C, M, N, M, Fl, M, Glm1, N, S, reg, KTUl, Y, part,
+          [=] __device__(long k, double s) {
+            Fbar[k] = s;
+            gA[k] += s / A[k];
+          },
+          nullptr);
+    }
+    launch_map(st, MS, [=] __device__(long k) { gA[k] *= reg; });
+    hipLaunchKernelGGL(k_colsum, dim3(S), dim3(kTB), 0, st, gwa, N, -1.0 / reg, gw);
+    check_launch();
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  return {fin.iter, fin.err};
+}
+
+}  // namespace rwig
+
+// ============================================================================
+// C ABI (include/rwig_hip.h)
+// ============================================================================
+
+using namespace rwig;
+
+namespace {
+
+void check_counts(int m, int n, int maxiter, double reg) {
+  RWIG_REQUIRE(m > 0 && n > 0, "matrix dimensions must be positive");
+  RWIG_REQUIRE(maxiter >= 0, "max_iter must be non-negative");
+  RWIG_REQUIRE(reg > 0.0, "reg must be positive");
+}
+
+template <class Core>
+int sinkhorn_abi(const char *name, Core core, double *P, double *grad_a, double *u, double *v,
+                 double *loss, int *iter_out, double *err_out, const double *a, const double *b,
+                 const double *C, int m, int n, double reg, int withgrad, int max_iter,
+                 double zero_tol, int *return_status_out, double off_support) {
+  return guarded(name, [&] {
+    check_counts(m, n, max_iter, reg);
+    RWIG_REQUIRE(a && b && C, "a, b and C are required");
+    ensure_device();
+    Reduced r = reduce_support(a, b, C, m, n);
+    RWIG_REQUIRE(r.M > 0 && r.N > 0, "a and b must each have a non-zero entry");
+    Stream s;
+    const long MN = (long)r.M * r.N;
+    DBuf<double> da(r.M), db(r.N), dC(MN), du(r.M), dv(r.N), dP(MN), dloss(1), dg(r.M);
+    h2d(da.get(), r.a.data(), r.M, s.get());
+    h2d(db.get(), r.b.data(), r.N, s.get());
+    h2d(dC.get(), r.C.data(), MN, s.get());
+    const SinkhornResult res =
+        core(s.get(), da.get(), db.get(), dC.get(), r.M, r.N, reg, withgrad != 0, max_iter,
+             zero_tol, du.get(), dv.get(), dP.get(), dloss.get(), dg.get());
+    std::vector<double> hu(r.M), hv(r.N), hP(P ? MN : 0), hg(withgrad ? r.M : 0);
+    double hl = 0.0;
+    d2h(hu.data(), du.get(), r.M, s.get());
+    d2h(hv.data(), dv.get(), r.N, s.get());
+    if (P) d2h(hP.data(), dP.get(), MN, s.get());
+    d2h(&hl, dloss.get(), 1, s.get());
+    if (withgrad) d2h(hg.data(), dg.get(), r.M, s.get());
+    s.sync();
+    if (u) {
+      for (int i = 0; i < m; ++i) u[i] = off_support;
+      for (int i = 0; i < r.M; ++i) u[r.ai[i]] = hu[i];
+    }
+    if (v) {
+      for (int j = 0; j < n; ++j) v[j] = off_support;
+      for (int j = 0; j < r.N; ++j) v[r.bi[j]] = hv[j];
+    }
+    if (P) {
+      std::fill(P, P + (size_t)m * n, 0.0);
+      for (int j = 0; j < r.N; ++j)
+        for (int i = 0; i < r.M; ++i) P[r.ai[i] + (size_t)r.bi[j] * m] = hP[i + (size_t)j * r.M];
+    }
+    if (withgrad && grad_a) {
+      std::fill(grad_a, grad_a + m, 0.0);
+      for (int i = 0; i < r.M; ++i) grad_a[r.ai[i]] = hg[i];
+    }
+    if (loss) *loss = hl;
+    if (iter_out) *iter_out = res.iter;
+    if (err_out) *err_out = res.err;
+    if (return_status_out)
+      *return_status_out = return_status(res.err, zero_tol, res.iter, max_iter);
+  });
+}
+
+template <class Core>
+int barycenter_abi(const char *name, Core core, double *U, double *V, double *b, double *grad_A,
+                   double *grad_w, double *loss, int *iter_out, double *err_out,
+                   const double *A, const double *w, const double *C, const double *b_ext, int M,
+                   int N, int S, double reg, int withgrad, int max_iter, double zero_tol,
+                   int *return_status_out) {
+  return guarded(name, [&] {
+    check_counts(M, N, max_iter, reg);
+    RWIG_REQUIRE(S > 0 && S <= 32, "S must be in [1, 32]");
+    RWIG_REQUIRE(A && w && C, "A, w and C are required");
+    RWIG_REQUIRE(!withgrad || b_ext, "b_ext is required with withgrad");
+    ensure_device();
+    Stream s;
+    const long MN = (long)M * N, MS = (long)M * S, NS = (long)N * S;
+    DBuf<double> dA(MS), dC(MN), dw(S), dbext(N), db(N), dU(MS), dV(NS), dgA(MS), dgw(S),
+        dloss(1);
+    h2d(dA.get(), A, MS, s.get());
+    h2d(dC.get(), C, MN, s.get());
+    h2d(dw.get(), w, S, s.get());
+    if (withgrad) h2d(dbext.get(), b_ext, N, s.get());
+    const SinkhornResult res =
+        core(s.get(), dA.get(), dC.get(), dw.get(), dbext.get(), M, N, S, reg, withgrad != 0,
+             max_iter, zero_tol, db.get(), dU.get(), dV.get(), dgA.get(), dgw.get(), dloss.get());
+    if (b) d2h(b, db.get(), N, s.get());
+    if (U) d2h(U, dU.get(), MS, s.get());
+    if (V) d2h(V, dV.get(), NS, s.get());
+    if (withgrad) {
+      if (grad_A) d2h(grad_A, dgA.get(), MS, s.get());
+      if (grad_w) d2h(grad_w, dgw.get(), S, s.get());
+      if (loss) d2h(loss, dloss.get(), 1, s.get());
+    }
+    s.sync();
+    if (iter_out) *iter_out = res.iter;
+    if (err_out) *err_out = res.err;
+    if (return_status_out)
+      *return_status_out = return_status(res.err, zero_tol, res.iter, max_iter);
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int rwig_sinkhorn_vanilla(double *P, double *grad_a, double *u, double *v, double *loss,
+                          int *iter_out, double *err_out, const double *a, const double *b,
+                          const double *C, int m, int n, double reg, int withgrad, int max_iter,
+                          double zero_tol, int *return_status_out) {
+  return sinkhorn_abi("rwig_sinkhorn_vanilla", sinkhorn_vanilla_dev, P, grad_a, u, v, loss,
+                      iter_out, err_out, a, b, C, m, n, reg, withgrad, max_iter, zero_tol,
+                      return_status_out, 0.0);
+}
+
+int rwig_sinkhorn_log(double *P, double *grad_a, double *f, double *g, double *loss,
+                      int *iter_out, double *err_out, const double *a, const double *b,
+                      const double *C, int m, int n, double reg, int withgrad, int max_iter,
+                      double zero_tol, int *return_status_out) {
+  return sinkhorn_abi("rwig_sinkhorn_log", sinkhorn_log_dev, P, grad_a, f, g, loss, iter_out,
+                      err_out, a, b, C, m, n, reg, withgrad, max_iter, zero_tol,
+                      return_status_out, -INFINITY);
+}
+
+int rwig_barycenter_parallel(double *U, double *V, double *b, double *grad_A, double *grad_w,
+                             double *loss, int *iter_out, double *err_out, const double *A,
+                             const double *w, const double *C, const double *b_ext, int M, int N,
+                             int S, double reg, int withgrad, int max_iter, double zero_tol,
+                             int *return_status_out) {
+  return barycenter_abi("rwig_barycenter_parallel", barycenter_parallel_dev, U, V, b, grad_A,
+                        grad_w, loss, iter_out, err_out, A, w, C, b_ext, M, N, S, reg, withgrad,
+                        max_iter, zero_tol, return_status_out);
+}
+
+int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A, double *grad_w,
+                        double *loss, int *iter_out, double *err_out, const double *A,
+                        const double *w, const double *C, const double *b_ext, int M, int N,
+                        int S, double reg, int withgrad, int max_iter, double zero_tol,
+                        int *return_status_out) {
+  return barycenter_abi("rwig_barycenter_log", barycenter_log_dev, F, G, b, grad_A, grad_w,
+                        loss, iter_out, err_out, A, w, C, b_ext, M, N, S, reg, withgrad,
+                        max_iter, zero_tol, return_status_out);
+}
+
+}  // extern "C"

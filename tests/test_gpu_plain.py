@@ -1,0 +1,187 @@
+"""GPU parity of the plain FP64 form (librwig_hip.so) against the reference's
+golden vectors and the FP64 oracle.
+
+Bar: the reference's tolerance (1.49e-7 mean-relative) on its golden
+vectors, and 1e-10 relative (max-abs over the array, scaled by its max) against
+the oracle on the same seeded inputs, with identical iteration counts.
+"""
+import numpy as np
+import pytest
+
+import rwig_amd
+from oracle import oracle as O
+from tests.conftest import mean_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_REF = 10 * np.sqrt(np.finfo(float).eps)
+TOL_FP64 = 1e-10
+
+
+def rel(x, y, floor=1e-6):
+    """max |x - y| over the finite entries, relative to max |y|; arrays whose
+    entries are all below `floor` (e.g. potentials that are zero up to
+    rounding) are compared on the scale `floor`."""
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    fin = np.isfinite(y)
+    assert np.array_equal(np.isfinite(x), fin)
+    if not fin.any():
+        return 0.0
+    scale = max(np.max(np.abs(y[fin])), floor)
+    return float(np.max(np.abs(x[fin] - y[fin])) / scale)
+
+
+def problem(seed, M, N, zeros=0, reg_scale=1.0):
+    rng = np.random.default_rng(seed)
+    a = rng.dirichlet(np.ones(M))
+    b = rng.dirichlet(np.ones(N))
+    if zeros:
+        a[rng.choice(M, zeros, replace=False)] = 0
+        b[rng.choice(N, zeros, replace=False)] = 0
+        a /= a.sum()
+        b /= b.sum()
+    x = rng.random((M, 2))
+    y = rng.random((N, 2))
+    C = ((x[:, None] - y[None]) ** 2).sum(-1)
+    return a, b, C
+
+
+def test_device_available():
+    assert rwig_amd.cuda_available()
+
+
+# ------------------------------------------------------------- golden vectors
+
+
+def test_golden_sinkhorn(golden):
+    g = golden["sinkhorn"]
+    a, b, C = np.array(g["a"]), np.array(g["b"]), np.array(g["C"])
+    sv = rwig_amd.sinkhorn(a, b, C, {"reg": g["reg"], "method": "vanilla", "with_grad": True,
+                                     "verbose": 0})
+    for k in ["loss", "P", "u", "v", "grad_a"]:
+        assert mean_rel(g["vanilla"][k], sv[k]) < TOL_REF, k
+    sl = rwig_amd.sinkhorn(a, b, C, {"reg": g["reg"], "method": "log", "with_grad": True,
+                                     "verbose": 0})
+    for k in ["loss", "P", "f", "g", "grad_a"]:
+        assert mean_rel(g["log"][k], sl[k]) < TOL_REF, k
+    assert sv["method"] == "vanilla" and sl["method"] == "log"
+
+
+def test_golden_barycenter(golden):
+    g = golden["barycenter"]
+    A, C, w, bext = (np.array(g[k]) for k in ["A", "C", "w", "b_ext"])
+    bp = rwig_amd.barycenter(A, C, w, bext, {"reg": g["reg"], "method": "parallel",
+                                             "with_grad": True, "verbose": 0})
+    for k in ["loss", "U", "V", "grad_A", "grad_w"]:
+        assert mean_rel(g["parallel"][k], bp[k]) < TOL_REF, k
+    bl = rwig_amd.barycenter(A, C, w, bext, {"reg": g["reg"], "method": "log",
+                                             "with_grad": True, "verbose": 0})
+    for k in ["loss", "grad_A", "grad_w"]:
+        assert mean_rel(g["log"][k], bl[k]) < TOL_REF, k
+
+
+# ------------------------------------------------------------- oracle parity
+
+
+@pytest.mark.parametrize("seed,M,N,zeros,reg,wg", [
+    (0, 5, 3, 0, 0.1, True),
+    (1, 40, 33, 3, 0.05, True),
+    (2, 300, 257, 10, 0.05, True),
+    (3, 1, 9, 0, 0.2, False),
+    (4, 700, 90, 0, 0.02, False),
+])
+def test_sinkhorn_vanilla_matches_oracle(seed, M, N, zeros, reg, wg):
+    a, b, C = problem(seed, M, N, zeros)
+    r = rwig_amd.sinkhorn(a, b, C, {"reg": reg, "method": "vanilla", "with_grad": wg})
+    o = O.sinkhorn_vanilla(a, b, C, reg, withgrad=wg)
+    assert r["iter"] == o["iter"]
+    assert r["return_status"] == o["return_status"]
+    keys = ["P", "u", "v", "loss"] + (["grad_a"] if wg else [])
+    for k in keys:
+        assert rel(r[k], o[k]) < TOL_FP64, k
+    assert abs(r["err"] - o["err"]) <= 1e-9 * max(o["err"], 1e-300) + 1e-16
+
+
+@pytest.mark.parametrize("seed,M,N,zeros,reg,wg", [
+    (10, 5, 3, 0, 0.1, True),
+    (11, 37, 41, 4, 0.05, True),
+    (12, 200, 300, 7, 0.01, True),
+    (13, 1000, 64, 0, 0.01, False),
+])
+def test_sinkhorn_log_matches_oracle(seed, M, N, zeros, reg, wg):
+    a, b, C = problem(seed, M, N, zeros)
+    r = rwig_amd.sinkhorn(a, b, C, {"reg": reg, "method": "log", "with_grad": wg})
+    o = O.sinkhorn_log(a, b, C, reg, withgrad=wg)
+    assert r["iter"] == o["iter"]
+    assert r["return_status"] == o["return_status"]
+    keys = ["P", "f", "g", "loss"] + (["grad_a"] if wg else [])
+    for k in keys:
+        assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+def test_sinkhorn_maxiter_exhaustion_status():
+    a, b, C = problem(20, 30, 20)
+    r = rwig_amd.sinkhorn(a, b, C, {"reg": 0.01, "method": "vanilla", "max_iter": 3})
+    assert r["iter"] == 3 and r["return_status"] == 1
+
+
+def bary_problem(seed, M, N, S):
+    rng = np.random.default_rng(seed)
+    A = rng.dirichlet(np.ones(M), size=S).T.copy()
+    w = rng.dirichlet(np.ones(S))
+    bext = rng.dirichlet(np.ones(N))
+    x = rng.random((M, 2))
+    y = rng.random((N, 2))
+    C = ((x[:, None] - y[None]) ** 2).sum(-1)
+    return A, C, w, bext
+
+
+@pytest.mark.parametrize("seed,M,N,S,reg,wg", [
+    (30, 5, 5, 2, 0.1, True),
+    (31, 50, 40, 3, 0.1, True),
+    (32, 256, 256, 8, 0.05, True),
+    (33, 100, 120, 5, 0.1, False),
+])
+def test_barycenter_parallel_matches_oracle(seed, M, N, S, reg, wg):
+    A, C, w, bext = bary_problem(seed, M, N, S)
+    r = rwig_amd.barycenter(A, C, w, bext if wg else None,
+                            {"reg": reg, "method": "parallel", "with_grad": wg})
+    o = O.barycenter_parallel(A, C, w, bext, reg, withgrad=wg)
+    assert r["iter"] == o["iter"]
+    keys = ["b", "U", "V"] + (["grad_A", "grad_w", "loss"] if wg else [])
+    for k in keys:
+        assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+@pytest.mark.parametrize("seed,M,N,S,reg,wg", [
+    (40, 5, 5, 2, 0.1, True),
+    (41, 40, 50, 3, 0.05, True),
+    (42, 200, 150, 4, 0.02, True),
+    (43, 64, 64, 1, 0.1, False),
+])
+def test_barycenter_log_matches_oracle(seed, M, N, S, reg, wg):
+    A, C, w, bext = bary_problem(seed, M, N, S)
+    r = rwig_amd.barycenter(A, C, w, bext if wg else None,
+                            {"reg": reg, "method": "log", "with_grad": wg})
+    o = O.barycenter_log(A, C, w, bext, reg, withgrad=wg)
+    assert r["iter"] == o["iter"]
+    keys = ["b", "F", "G"] + (["grad_A", "grad_w", "loss"] if wg else [])
+    for k in keys:
+        assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+def test_c1_vanilla_256_config():
+    # 1-D grid x_i = i/255, squared distance (max 1), Dirichlet(1) floored at
+    # 1e-12, eps 0.05, tol 1e-9, up to 1000 iterations
+    rng = np.random.default_rng(1001)
+    x = np.arange(256) / 255.0
+    C = (x[:, None] - x[None]) ** 2
+    a = np.maximum(rng.dirichlet(np.ones(256)), 1e-12); a /= a.sum()
+    b = np.maximum(rng.dirichlet(np.ones(256)), 1e-12); b /= b.sum()
+    r = rwig_amd.sinkhorn(a, b, C, {"reg": 0.05, "method": "vanilla", "zero_tol": 1e-9,
+                                    "with_grad": True})
+    o = O.sinkhorn_vanilla(a, b, C, 0.05, withgrad=True, zerotol=1e-9)
+    assert r["iter"] == o["iter"]
+    for k in ["P", "u", "v", "loss", "grad_a"]:
+        assert rel(r[k], o[k]) < TOL_FP64, k

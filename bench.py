@@ -1,0 +1,215 @@
+"""Benchmark of the entropic-OT hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): batched
+log-stabilised Sinkhorn, one source against many targets -- n = 4096 points
+uniform in [0,1]^2, squared Euclidean cost, uniform source, B = 1024 target
+histograms with Dirichlet(0.5) mass on a random 5% support (205 points),
+eps = 0.01, every solve run until its L1 marginal error
+sum|P1 - a| + sum|P^T 1 - b| (FP64, on the GPU) is <= 1e-6.
+
+A step is one batched solve of all 1024 targets with inputs resident in HBM.
+The headline value is BF16 solves/s (BF16 Gibbs kernel, FP32 iterations, FP64
+absorption every 20 iterations); FP32, the batched FP64 log-domain form and
+the plain FP64 per-call form (one rwig_sinkhorn_log_dev call per target) are
+measured beside it, and the CPU oracle (the reference algorithm restated in
+C++) is timed on a bounded sample of targets on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--targets", type=int, default=1024)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--cpu-sample", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched "
+              "solves shard by target with no exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from rwig_amd import batched as RB
+    from rwig_amd import workloads
+    from rwig_amd._lib import check, lib
+
+    tol = 1e-6
+    log("generating the workload")
+    C, a, sup, wts, reg = workloads.c2(B=args.targets)
+    M = C.shape[0]
+    B = len(sup)
+    nts = np.array([len(s) for s in sup])
+    dev = torch.device("cuda:0")
+    Cd = torch.from_numpy(C).to(dev)
+    assert Cd.stride() == (1, M)
+    ad = torch.from_numpy(a).to(dev)
+    tb = RB.TargetBatch(sup, wts, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def run(precision, Klo=None, out=None):
+        return RB.sinkhorn_batched(Cd, ad, tb, reg, precision=precision, max_iter=1000,
+                                   zero_tol=tol, err="l1", absorb_every=20, Klo=Klo, out=out)
+
+    def check_out(out, name):
+        torch.cuda.synchronize()
+        err = out["err"].cpu().numpy()
+        it = out["iter"].cpu().numpy()
+        ok = bool(np.all(err <= tol))
+        log(f"{name}: iterations min/mean/max {it.min()}/{it.mean():.1f}/{it.max()}, "
+            f"max err {err.max():.3e}, all converged {ok}")
+        return ok, it, err
+
+    # ---------------------------------------------------------------- BF16 headline
+    log("building the BF16 Gibbs kernel")
+    Klo = RB.gibbs_lowp(Cd, reg, "bf16")
+    out = run("bf16", Klo)
+    for _ in range(args.warmup):
+        run("bf16", Klo, out)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        run("bf16", Klo, out)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
+    ok_bf16, it_bf16, _ = check_out(out, "bf16")
+    value = B * args.steps / wall
+    kern_ms = float(step_ms.mean())
+
+    # algorithmic bytes of one launch of the resident BF16 kernel: per target,
+    # every iteration streams its nt columns of the BF16 kernel (M x nt x 2 B)
+    # and every absorption its nt cost columns in FP64 (M x nt x 8 B)
+    n_abs = np.ceil(it_bf16 / 20.0) + 1
+    alg_bytes = float(np.sum(it_bf16 * nts * M * 2.0 + n_abs * nts * M * 8.0))
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+
+    # ---------------------------------------------------------------- FP32 and FP64 batched
+    def timed(precision, Klo=None, reps=1):
+        o = run(precision, Klo)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            run(precision, Klo, o)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        ok, it, _ = check_out(o, precision)
+        return B / dt, ok, it
+
+    Klo32 = RB.gibbs_lowp(Cd, reg, "fp32")
+    fp32_sps, ok32, it32 = timed("fp32", Klo32, reps=2)
+    fp64_sps, ok64, it64 = timed("fp64", reps=1)
+
+    # ---------------------------------------------------------------- plain FP64 per call
+    log("plain FP64 per-call form on a sample of targets")
+    L = lib()
+    ns = min(args.plain_sample, B)
+    t_plain = 0.0
+    for t in range(ns):
+        idx = torch.as_tensor(sup[t], device=dev, dtype=torch.long)
+        Ct = Cd.index_select(1, idx).t().contiguous().t()
+        bt = torch.as_tensor(wts[t], device=dev)
+        P = torch.empty((M, len(sup[t])), dtype=torch.float64, device=dev)
+        f = torch.empty(M, dtype=torch.float64, device=dev)
+        g = torch.empty(len(sup[t]), dtype=torch.float64, device=dev)
+        lo = torch.empty(1, dtype=torch.float64, device=dev)
+        it = ctypes.c_int()
+        er = ctypes.c_double()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        check(L.rwig_sinkhorn_log_dev(ad.data_ptr(), bt.data_ptr(), Ct.data_ptr(), M,
+                                      len(sup[t]), reg, 0, 1000, tol, P.data_ptr(), None,
+                                      f.data_ptr(), g.data_ptr(), lo.data_ptr(),
+                                      ctypes.byref(it), ctypes.byref(er),
+                                      stream.cuda_stream))
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t1
+    plain_sps = ns / t_plain
+    log(f"plain FP64 per call: {plain_sps:.2f} solves/s over {ns} targets")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu and args.cpu_sample > 0:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        nc = min(args.cpu_sample, B)
+        log(f"CPU oracle (log Sinkhorn, {cores} threads) on {nc} targets")
+        t1 = time.perf_counter()
+        for t in range(nc):
+            Ct = np.asfortranarray(C[:, sup[t]])
+            O.sinkhorn_log(a, wts[t], Ct, reg, n_threads=cores, maxiter=1000, zerotol=tol)
+        dt = time.perf_counter() - t1
+        cpu = {"value": nc / dt, "unit": "solves/s", "cores": cores, "kind": "port",
+               "sample": f"{nc} of the {B} targets, reference log-Sinkhorn algorithm "
+                         f"(C++ oracle, n_threads={cores}, zero_tol=1e-6 on the "
+                         "reference error), n=4096 x 205 reduced problems"}
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(step_ms, 50)),
+        "p99_ms": float(np.percentile(step_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched log-stabilised Sinkhorn, one source vs many targets",
+            "n": M, "targets": B, "support": int(nts.max()), "eps": reg,
+            "stop": "L1 marginal error <= 1e-6 (FP64 on GPU)",
+            "precision": "BF16 Gibbs kernel, FP32 iterations, FP64 absorption every 20",
+        },
+        "converged": {"bf16": ok_bf16, "fp32": ok32, "fp64": ok64},
+        "iterations_mean": {"bf16": float(it_bf16.mean()), "fp32": float(it32.mean()),
+                            "fp64": float(it64.mean())},
+        "modes_solves_per_s": {"bf16": value, "fp32": fp32_sps, "fp64_batched": fp64_sps,
+                               "fp64_plain_per_call": plain_sps},
+        "speedup_vs_plain_fp64": value / plain_sps,
+        "speedup_vs_fp64_batched": value / fp64_sps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sinkhorn_fast_resident<bf16>", "kernel_ms": kern_ms,
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

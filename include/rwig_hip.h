@@ -83,6 +83,55 @@ int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A,
                         int S, double reg, int withgrad, int max_iter,
                         double zero_tol, int *return_status);
 
+/* rwig_sinkhorn_log on DEVICE pointers of an already reduced problem (every
+ * a_i and b_j > 0): the per-call plain FP64 form with inputs resident in HBM.
+ * P (m x n), f (m), g (n), loss (1 scalar) and grad_a (m, withgrad) are
+ * device outputs; the call synchronises the stream before returning. */
+int rwig_sinkhorn_log_dev(const double *a, const double *b, const double *C,
+                          int m, int n, double reg, int withgrad, int max_iter,
+                          double zero_tol, double *P, double *grad_a, double *f,
+                          double *g, double *loss, int *iter_out,
+                          double *err_out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched solves, device pointers (no counterpart in the reference, whose GPU
+ * path solves one problem per call, inst/include/cuda_interface.cuh:13-39; the
+ * reference solves a batch as repeated sinkhorn_log_cpp calls,
+ * src/sinkhorn.cpp:142).
+ *
+ * One source measure a (M points, all a_i > 0) against B target measures.
+ * C is the FP64 cost, column-major, leading dimension ldc; target t is
+ * supported on the columns col_idx[col_ptr[t] .. col_ptr[t+1]) of C with
+ * masses b[] at the same positions (all > 0); max_nt bounds the support size.
+ * Outputs: f (B x M, target-major), g (at the positions of b), iters (B),
+ * errs (B) and, if loss is non-null, loss[2t] = sum C.*P and
+ * loss[2t+1] = sum C.*P + reg sum P.*(log P - 1).
+ *
+ * precision 0: FP64 log-domain iterations of Sinkhorn::_fwd_log
+ *   (src/sinkhorn_impl_log.cpp:298-376), one kernel launch per batch;
+ * precision 1 / 2: FP64 absorption of the scalings into (f, g) every
+ *   absorb_every iterations plus defect-corrected iterations on a shared
+ *   low-precision Gibbs kernel Klo (FP32 / BF16, see rwig_gibbs_lowp_dev);
+ *   the stopping test runs at the absorptions on the exact FP64 plan.
+ * err_mode 0: the reference's error ||log-marginal residuals||_2 (summed over
+ *   rows and columns, src/sinkhorn_impl_log.cpp:360); 1: the L1 marginal error
+ *   sum|P1 - a| + sum|P^T 1 - b|.
+ */
+int rwig_sinkhorn_batched_dev(const double *C, long ldc, int M, const double *a,
+                              int B, const int *col_ptr, const int *col_idx,
+                              const double *b, int max_nt, double reg,
+                              int max_iter, double zero_tol, int err_mode,
+                              int precision, const void *Klo, long ldk,
+                              int absorb_every, double *f, double *g,
+                              int *iters, double *errs, double *loss,
+                              void *stream);
+
+/* Klo = exp(-C/reg) rounded to FP32 (precision 1) or BF16 (precision 2),
+ * column-major M x N with leading dimension ldk (>= M, multiple of 4; rows
+ * M .. ldk-1 are zero). Built once per cost matrix and reg. */
+int rwig_gibbs_lowp_dev(const double *C, long ldc, int M, int N, double reg,
+                        int precision, void *Klo, long ldk, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,12 @@ SIGNATURES = {
                                   _i, _i, _i, _d, _i, _i, _d, _ip], _i),
     "rwig_barycenter_log": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _i,
                              _i, _i, _d, _i, _i, _d, _ip], _i),
+    "rwig_sinkhorn_batched_dev": ([_vp, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp, _i, _d, _i,
+                                   _d, _i, _i, _vp, ctypes.c_long, _i, _vp, _vp, _vp, _vp, _vp,
+                                   _vp], _i),
+    "rwig_sinkhorn_log_dev": ([_vp, _vp, _vp, _i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp,
+                               _ip, _dp, _vp], _i),
+    "rwig_gibbs_lowp_dev": ([_vp, ctypes.c_long, _i, _i, _d, _i, _vp, ctypes.c_long, _vp], _i),
 }
 
 _lib = None

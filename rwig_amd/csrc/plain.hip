@@ -978,4 +978,20 @@ int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A, double 
                         max_iter, zero_tol, return_status_out);
 }
 
+int rwig_sinkhorn_log_dev(const double *a, const double *b, const double *C, int m, int n,
+                          double reg, int withgrad, int max_iter, double zero_tol, double *P,
+                          double *grad_a, double *f, double *g, double *loss, int *iter_out,
+                          double *err_out, void *stream) {
+  return guarded("rwig_sinkhorn_log_dev", [&] {
+    check_counts(m, n, max_iter, reg);
+    RWIG_REQUIRE(a && b && C && P && f && g && loss, "null argument");
+    RWIG_REQUIRE(!withgrad || grad_a, "grad_a is required with withgrad");
+    const SinkhornResult res =
+        sinkhorn_log_dev((hipStream_t)stream, a, b, C, m, n, reg, withgrad != 0, max_iter,
+                         zero_tol, f, g, P, loss, grad_a);
+    if (iter_out) *iter_out = res.iter;
+    if (err_out) *err_out = res.err;
+  });
+}
+
 }  // extern "C"

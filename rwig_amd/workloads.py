@@ -1,0 +1,40 @@
+"""Seeded synthetic inputs of the benchmark configurations (seed = 1000 + id).
+
+C1: vanilla Sinkhorn, n = m = 256 on the 1-D grid x_i = i/255, squared
+    distance (max 1), a, b ~ Dirichlet(1) floored at 1e-12, eps 0.05.
+C2: one source against many targets: n = 4096 points uniform in [0,1]^2,
+    squared Euclidean cost, a uniform, B = 1024 targets each with
+    Dirichlet(0.5) mass on a random 5% support (205 points), eps 0.01.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def c1(seed=1001):
+    rng = np.random.default_rng(seed)
+    x = np.arange(256) / 255.0
+    C = (x[:, None] - x[None]) ** 2
+    a = np.maximum(rng.dirichlet(np.ones(256)), 1e-12)
+    b = np.maximum(rng.dirichlet(np.ones(256)), 1e-12)
+    return a / a.sum(), b / b.sum(), np.asfortranarray(C), 0.05
+
+
+def c2(n=4096, B=1024, frac=0.05, seed=1002):
+    """Returns (C (n x n, Fortran order), a, supports, weights, eps)."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 2))
+    sq = (X * X).sum(1)
+    C = sq[:, None] + sq[None, :] - 2.0 * X @ X.T
+    np.maximum(C, 0.0, out=C)
+    np.fill_diagonal(C, 0.0)
+    a = np.full(n, 1.0 / n)
+    k = max(1, int(round(frac * n)))
+    supports, weights = [], []
+    for _ in range(B):
+        s = np.sort(rng.choice(n, k, replace=False))
+        w = rng.dirichlet(np.full(k, 0.5))
+        w = np.maximum(w, 1e-300)
+        supports.append(s)
+        weights.append(w / w.sum())
+    return np.asfortranarray(C), a, supports, weights, 0.01

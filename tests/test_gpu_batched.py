@@ -1,0 +1,130 @@
+"""GPU tests of the batched one-source-many-targets Sinkhorn
+(rwig_sinkhorn_batched_dev).
+
+FP64 mode: every target equals the oracle's log Sinkhorn on the target's
+reduced problem (src/sinkhorn_impl_log.cpp), same iteration count, f and g to
+1e-10 relative. FP32 / BF16 modes: the returned potentials define a plan whose
+L1 marginal error, recomputed here in FP64 on the host, is below the stated
+tolerance, and whose transport cost agrees with the FP64 mode's to 1e-5
+relative. Every mode gives bit-identical results for a target solved alone and
+inside a batch.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from rwig_amd import batched as RB  # noqa: E402
+from rwig_amd import workloads  # noqa: E402
+
+
+def make(seed, M, B, nt_lo, nt_hi, reg):
+    rng = np.random.default_rng(seed)
+    X = rng.random((M, 2))
+    C = np.asfortranarray(((X[:, None] - X[None]) ** 2).sum(-1))
+    a = rng.dirichlet(np.ones(M))
+    sup, wts = [], []
+    for _ in range(B):
+        k = int(rng.integers(nt_lo, nt_hi + 1))
+        sup.append(np.sort(rng.choice(M, k, replace=False)))
+        wts.append(rng.dirichlet(np.full(k, 0.5)))
+    return C, a, sup, wts, reg
+
+
+def to_dev(C, a):
+    Ct = torch.from_numpy(np.asfortranarray(C)).cuda()
+    Ct = Ct.t().contiguous().t()  # keep column-major strides (1, M)
+    assert Ct.stride() == (1, C.shape[0])
+    return Ct, torch.from_numpy(a).cuda()
+
+
+def host(out, tb):
+    torch.cuda.synchronize()
+    f = out["f"].cpu().numpy()
+    g = out["g"].cpu().numpy()
+    gs = [g[tb.offsets[t]:tb.offsets[t + 1]] for t in range(tb.B)]
+    return f, gs, out["iter"].cpu().numpy(), out["err"].cpu().numpy(), out["loss"].cpu().numpy()
+
+
+def plan_l1(C, a, sup, w, f, g, reg):
+    Ct = C[:, sup]
+    P = np.exp(-((Ct - f[:, None]) - g[None, :]) / reg)
+    return np.abs(P.sum(1) - a).sum() + np.abs(P.sum(0) - w).sum(), (Ct * P).sum()
+
+
+@pytest.mark.parametrize("seed,M,B,lo,hi,reg", [
+    (1, 300, 5, 8, 60, 0.05),
+    (2, 1000, 3, 40, 90, 0.02),
+    (3, 64, 4, 1, 64, 0.1),
+])
+def test_fp64_batched_matches_oracle_per_target(seed, M, B, lo, hi, reg):
+    C, a, sup, wts, reg = make(seed, M, B, lo, hi, reg)
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    out = RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64", err="reference")
+    f, gs, it, err, loss = host(out, tb)
+    for t in range(B):
+        print("target", t, "nt", len(sup[t]), flush=True)
+        bfull = np.zeros(M)
+        bfull[sup[t]] = wts[t]
+        o = O.sinkhorn_log(a, bfull, C, reg)
+        assert it[t] == o["iter"]
+        scale = max(np.abs(o["f"]).max(), 1e-3)
+        assert np.abs(f[t] - o["f"]).max() / scale < 1e-10
+        og = o["g"][sup[t]]
+        assert np.abs(gs[t] - og).max() / scale < 1e-10
+        assert abs(loss[t, 1] - o["loss"]) <= 1e-10 * abs(o["loss"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_fast_modes_reach_tolerance_and_match_fp64(precision):
+    C, a, sup, wts, _ = make(7, 2048, 6, 80, 120, 0.01)
+    reg, tol = 0.01, 1e-6
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    ref = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64", err="l1",
+                                   zero_tol=tol, max_iter=5000), tb)
+    out = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision=precision, err="l1",
+                                   zero_tol=tol, max_iter=5000, absorb_every=20), tb)
+    for t in range(tb.B):
+        e_ref, c_ref = plan_l1(C, a, sup[t], wts[t], ref[0][t], ref[1][t], reg)
+        e, c = plan_l1(C, a, sup[t], wts[t], out[0][t], out[1][t], reg)
+        print(precision, t, "iters", out[2][t], "vs", ref[2][t], "err", e, flush=True)
+        assert e_ref <= tol * 1.0001
+        assert e <= tol * 1.0001, (t, e)
+        assert abs(out[3][t] - e) <= 1e-9 + 1e-6 * e  # reported error is the plan's
+        assert abs(c - c_ref) <= 1e-5 * abs(c_ref)
+        assert abs(out[4][t, 0] - c) <= 1e-9 * abs(c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_result_alone_equals_result_in_mix(precision):
+    C, a, sup, wts, reg = make(11, 512, 7, 5, 50, 0.02)
+    Cd, ad = to_dev(C, a)
+    mix = host(RB.sinkhorn_batched(Cd, ad, RB.TargetBatch(sup, wts), reg, precision=precision,
+                                   err="l1"), RB.TargetBatch(sup, wts))
+    for t in [0, 3, 6]:
+        tb1 = RB.TargetBatch([sup[t]], [wts[t]])
+        one = host(RB.sinkhorn_batched(Cd, ad, tb1, reg, precision=precision, err="l1"), tb1)
+        assert np.array_equal(one[0][0], mix[0][t])
+        assert np.array_equal(one[1][0], mix[1][t])
+        assert one[2][0] == mix[2][t]
+        assert one[3][0] == mix[3][t]
+
+
+def test_c2_scaled_down_fp64_against_oracle():
+    # the C2 generator at a quarter of its size, two targets through the oracle
+    C, a, sup, wts, reg = workloads.c2(n=1024, B=4)
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    f, gs, it, err, loss = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64",
+                                                    err="reference"), tb)
+    for t in range(2):
+        bfull = np.zeros(C.shape[0])
+        bfull[sup[t]] = wts[t]
+        o = O.sinkhorn_log(a, bfull, C, reg)
+        assert it[t] == o["iter"]
+        assert np.abs(f[t] - o["f"]).max() / np.abs(o["f"]).max() < 1e-10

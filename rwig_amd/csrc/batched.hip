@@ -69,7 +69,7 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
       rmn[k] = INFINITY;
       rsm[k] = 0.0;
     }
-    double colerr = 0.0;
+    double colacc = 0.0;  // lanes 0..CB-1: error terms of their columns
     for (int jb = 0; jb < nt; jb += CB) {
       double c[RPT][CB], gq[CB], lbq[CB], bq[CB];
 #pragma unroll
@@ -87,65 +87,63 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
         }
       }
       if (update_g) {
-        // column minima of R(f, g)
-        double v[CB];
+        // column minima of R(f, g): per-wave minima, then lanes 0..CB-1 of
+        // the workgroup finish one column each
+        double *mS = red + 2 * kBW * CB;  // CB column minima
+        double *gS2 = mS + CB;            // CB new g values
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
           double m = INFINITY;
 #pragma unroll
           for (int k = 0; k < RPT; ++k)
             if (own[k]) m = fmin(m, (c[k][q] - f[k]) - gq[q]);
-          v[q] = wave_min(m);
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int q = 0; q < CB; ++q) red[wv * CB + q] = v[q];
+          m = wave_min(m);
+          if (lane == 0) red[wv * CB + q] = m;
         }
         __syncthreads();
-        double mn[CB];
-#pragma unroll
-        for (int q = 0; q < CB; ++q) {
+        if (threadIdx.x < CB) {
+          const int q = threadIdx.x;
           double m = red[q];
           for (int w = 1; w < kBW; ++w) m = fmin(m, red[w * CB + q]);
-          mn[q] = m;
+          mS[q] = m;
         }
+        __syncthreads();
         // column sums of exp(-(R - min) / reg)
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
-          double s = 0.0;
+          const double mn = mS[q];
+          double sacc = 0.0;
 #pragma unroll
           for (int k = 0; k < RPT; ++k)
-            if (own[k]) s += exp(-(((c[k][q] - f[k]) - gq[q]) - mn[q]) / reg);
-          v[q] = wave_sum(s);
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int q = 0; q < CB; ++q) red[kBW * CB + wv * CB + q] = v[q];
+            if (own[k]) sacc += exp(-(((c[k][q] - f[k]) - gq[q]) - mn) / reg);
+          sacc = wave_sum(sacc);
+          if (lane == 0) red[kBW * CB + wv * CB + q] = sacc;
         }
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < CB; ++q) {
+        if (threadIdx.x < CB) {
+          const int q = threadIdx.x;
+          double gn = gq[q];
           if (jb + q < nt) {
-            double s = 0.0;
-            for (int w = 0; w < kBW; ++w) s += red[kBW * CB + w * CB + q];
-            const double rmc = mn[q] - reg * log(s);          // Rmincol of R(f, g)
-            const double gn = gq[q] + (reg * lbq[q] + rmc);   // g update
-            const double rmc2 = rmc - (gn - gq[q]);           // Rmincol of R(f, g_new)
+            double sacc = 0.0;
+            for (int w = 0; w < kBW; ++w) sacc += red[kBW * CB + w * CB + q];
+            const double rmc = mS[q] - reg * log(sacc);     // Rmincol of R(f, g)
+            gn = gq[q] + (reg * lbq[q] + rmc);              // g update
+            const double rmc2 = rmc - (gn - gq[q]);         // Rmincol of R(f, g_new)
+            double e;
             if (err_mode == 0) {
               const double r = -rmc2 / reg - lbq[q];
-              colerr += r * r;
+              e = r * r;
             } else {
-              colerr += fabs(exp(-rmc2 / reg) - bq[q]);
+              e = fabs(exp(-rmc2 / reg) - bq[q]);
             }
-            gq[q] = gn;
+            colacc += e;
+            gs[jb + q] = gn;
           }
+          gS2[q] = gn;
         }
-        __syncthreads();  // red is reused by the next chunk
-        if (threadIdx.x == 0) {
+        __syncthreads();  // also orders the reuse of red by the next chunk
 #pragma unroll
-          for (int q = 0; q < CB; ++q)
-            if (jb + q < nt) gs[jb + q] = gq[q];
-        }
+        for (int q = 0; q < CB; ++q) gq[q] = gS2[q];
       }
       // online row soft-min of R(f, g) over this chunk
 #pragma unroll
@@ -166,7 +164,7 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
 #pragma unroll
     for (int k = 0; k < RPT; ++k) rmin[k] = own[k] ? rmn[k] - reg * log(rsm[k]) : 0.0;
     __syncthreads();
-    return colerr;
+    return colacc;
   };
 
   auto block_total = [&](double v) -> double {
@@ -187,7 +185,7 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
 #pragma unroll
     for (int k = 0; k < RPT; ++k)
       if (own[k]) f[k] = f[k] + (reg * log(a[threadIdx.x + k * kBTB]) + rmin[k]);
-    const double colerr = sweep(true);
+    const double colpart = sweep(true);
     double rowerr = 0.0;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -200,7 +198,7 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
       }
     }
     const double rt = block_total(rowerr);
-    const double ct = colerr;  // computed identically by every thread
+    const double ct = block_total(colpart);
     err = (err_mode == 0) ? sqrt(rt) + sqrt(ct) : rt + ct;
   }
 #pragma unroll
@@ -246,7 +244,7 @@ void sinkhorn_log_batched_fp64(hipStream_t st, const double *C, long ldc, int M,
                                int err_mode, double *F, double *G, int *iters, double *errs,
                                double *loss) {
   RWIG_REQUIRE(M >= 1 && M <= 16 * kBTB, "batched solve supports 1..8192 source points");
-  const size_t shmem = sizeof(double) * ((size_t)max_nt + 2 * kBW * 8);
+  const size_t shmem = sizeof(double) * ((size_t)max_nt + 2 * kBW * 8 + 2 * 8);
   RWIG_REQUIRE(max_nt >= 1 && shmem <= 65536, "targets must have 1..8000 support points");
   const dim3 grid(B), block(kBTB);
 #define RWIG_LAUNCH_LOG(RPT_, CB_)                                                            \

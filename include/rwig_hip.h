@@ -132,6 +132,43 @@ int rwig_sinkhorn_batched_dev(const double *C, long ldc, int M, const double *a,
 int rwig_gibbs_lowp_dev(const double *C, long ldc, int M, int N, double reg,
                         int precision, void *Klo, long ldk, void *stream);
 
+/* Batched Sinkhorn on MFMA (one source against B targets sharing one cost
+ * matrix): every Gibbs product of an iteration is one BF16 MFMA GEMM over all
+ * targets (FP32 accumulation), with the scaling update fused into its
+ * epilogue; FP64 absorption every absorb_every iterations computes the exact
+ * marginals by FP64 MFMA GEMMs and decides convergence (L1 or reference
+ * error <= zero_tol). Targets whose absorbed error is above switch_thr
+ * iterate in a multiplicative form (positive sums only), the others in an
+ * additive defect-correction form whose fixed point is the FP64 one.
+ * The kernel handle holds exp(-C/reg) in BF16 and FP64 in both orientations
+ * (rwig_dense_kernel_create, one per cost matrix and reg). tgt_of[p] is the
+ * target owning support entry p. Other arguments as rwig_sinkhorn_batched_dev.
+ * No counterpart in the reference (it solves one problem per call). */
+int rwig_dense_kernel_create(const double *C, long ldc, int M, int Nc,
+                             double reg, void *stream, void **handle);
+int rwig_dense_kernel_destroy(void *handle);
+int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc,
+                                   const double *a, int B, const int *col_ptr,
+                                   const int *col_idx, const int *tgt_of,
+                                   int nnz, const double *b, double reg,
+                                   int max_iter, double zero_tol, int err_mode,
+                                   int absorb_every, double switch_thr,
+                                   double *f, double *g, int *iters,
+                                   double *errs, double *loss, void *stream);
+
+/* MFMA GEMM D(m, n) = sum_k A(m, k) B(k, n) with A row-major (A[m*lda + k])
+ * and B, D column-major (B[k + n*ldb], D[m + n*ldd]); lda and ldb multiples
+ * of 16 bytes. The building block of the batched Gibbs products (no
+ * counterpart in the reference, which calls cuBLAS dgemm,
+ * src/cuda_kernels.cu:563). bf16: A, B are BF16 bit patterns, D is FP32
+ * (v_mfma_f32_16x16x32_bf16); f64: all FP64 (v_mfma_f64_16x16x4_f64). */
+int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda,
+                          const void *B, long ldb, float *D, long ldd,
+                          void *stream);
+int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
+                         const double *B, long ldb, double *D, long ldd,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,16 @@ SIGNATURES = {
                                    _vp], _i),
     "rwig_sinkhorn_log_dev": ([_vp, _vp, _vp, _i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp,
                                _ip, _dp, _vp], _i),
+    "rwig_gemm_tn_bf16_dev": ([_i, _i, _i, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp,
+                               ctypes.c_long, _vp], _i),
+    "rwig_gemm_tn_f64_dev": ([_i, _i, _i, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp,
+                              ctypes.c_long, _vp], _i),
+    "rwig_dense_kernel_create": ([_vp, ctypes.c_long, _i, _i, _d, _vp,
+                                  ctypes.POINTER(ctypes.c_void_p)], _i),
+    "rwig_dense_kernel_destroy": ([_vp], _i),
+    "rwig_sinkhorn_batched_mfma_dev": ([_vp, _vp, ctypes.c_long, _vp, _i, _vp, _vp, _vp, _i, _vp,
+                                        _d, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp],
+                                       _i),
     "rwig_gibbs_lowp_dev": ([_vp, ctypes.c_long, _i, _i, _d, _i, _vp, ctypes.c_long, _vp], _i),
 }
 

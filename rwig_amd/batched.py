@@ -8,6 +8,8 @@ that call on its reduced problem (the support of b), solved side by side.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -50,6 +52,8 @@ class TargetBatch:
         self.nnz = int(ptr[-1])
         self.col_ptr = torch.as_tensor(ptr, device=device)
         self.col_idx = torch.as_tensor(idx, device=device)
+        self.tgt_of = torch.as_tensor(np.repeat(np.arange(self.B, dtype=np.int32), sizes),
+                                      device=device)
         self.b = torch.as_tensor(val, device=device)
         self.offsets = ptr
 
@@ -95,4 +99,57 @@ def sinkhorn_batched(C, a, targets: TargetBatch, reg, precision="fp64", max_iter
         float(zero_tol), ERR_MODES[err], prec, _ptr(kt), ldk, int(absorb_every),
         _ptr(out["f"]), _ptr(out["g"]), _ptr(out["iter"]), _ptr(out["err"]),
         _ptr(out["loss"]), _stream_ptr(stream)))
+    return out
+
+
+class DenseKernel:
+    """exp(-C/reg) in BF16 and FP64, both orientations, for the MFMA path
+    (rwig_dense_kernel_create). Built once per cost matrix and reg."""
+
+    def __init__(self, C, reg, stream=None):
+        if C.dtype != torch.float64 or C.stride(0) != 1:
+            raise ValueError("C must be a column-major float64 device tensor")
+        self.M, self.Nc = C.shape
+        self.reg = float(reg)
+        h = ctypes.c_void_p()
+        check(lib().rwig_dense_kernel_create(_ptr(C), C.stride(1), self.M, self.Nc, self.reg,
+                                             _stream_ptr(stream), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().rwig_dense_kernel_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sinkhorn_batched_mfma(C, a, targets: TargetBatch, reg, kernel: DenseKernel = None,
+                          max_iter=1000, zero_tol=1e-6, err="l1", absorb_every=20,
+                          switch_thr=1e-2, with_loss=True, stream=None, out=None):
+    """Batched solve with every Gibbs product on BF16 MFMA (all targets as GEMM
+    columns), FP64 absorption every absorb_every iterations."""
+    if kernel is None:
+        kernel = DenseKernel(C, reg, stream)
+    M = C.shape[0]
+    dev = C.device
+    if out is None:
+        out = {
+            "f": torch.empty((targets.B, M), dtype=torch.float64, device=dev),
+            "g": torch.empty(targets.nnz, dtype=torch.float64, device=dev),
+            "iter": torch.empty(targets.B, dtype=torch.int32, device=dev),
+            "err": torch.empty(targets.B, dtype=torch.float64, device=dev),
+            "loss": torch.empty((targets.B, 2), dtype=torch.float64, device=dev)
+            if with_loss else None,
+        }
+    check(lib().rwig_sinkhorn_batched_mfma_dev(
+        kernel.handle, _ptr(C), C.stride(1), _ptr(a), targets.B, _ptr(targets.col_ptr),
+        _ptr(targets.col_idx), _ptr(targets.tgt_of), targets.nnz, _ptr(targets.b), float(reg),
+        int(max_iter), float(zero_tol), ERR_MODES[err], int(absorb_every), float(switch_thr),
+        _ptr(out["f"]), _ptr(out["g"]), _ptr(out["iter"]), _ptr(out["err"]), _ptr(out["loss"]),
+        _stream_ptr(stream)))
     return out

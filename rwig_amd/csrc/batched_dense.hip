@@ -1,0 +1,373 @@
+// Batched Sinkhorn on MFMA: one source against B targets that share one cost
+// matrix, with every Gibbs product of an iteration done as one dense GEMM
+// over all targets (targets are the GEMM columns).
+//
+// Notation: K = exp(-C/reg) (M x Nc), target t supported on S_t subset of
+// the Nc columns. Potentials f (M x B), g (on the supports) in FP64;
+// alpha = exp(f/reg), beta = exp(g/reg); the absorbed plan of target t is
+// P0_ij = alpha_i K_ij beta_j and the current plan P = diag(u) P0 diag(v).
+//
+// Per iteration (BF16 MFMA, FP32 accumulation, Sinkhorn update fused into the
+// GEMM epilogue):
+//   Z = K^T W  ->  v on the supports, X = beta .* (v or v - 1) (zero off S_t)
+//   Y = K X    ->  u,                 W = alpha .* (u or u - 1)
+// Two forms of the update, chosen per target at every absorption:
+//   multiplicative (far from convergence):  v = b / (sb .* z), u = a / (ra .* y)
+//     with W = alpha u, X = beta v and calibration ratios ra = r0 / (K beta),
+//     sb = c0 / (K^T alpha) computed with the same BF16 arithmetic at u = v = 1,
+//     so the rounding of K divides out; all sums are of positive terms.
+//   additive (near convergence):  v = b / (c0 + beta .* z), u = a / (r0 + alpha .* y)
+//     with W = alpha (u - 1), X = beta (v - 1): the exact FP64 marginals r0, c0
+//     carry the O(1) part and the BF16 products only the corrections, whose
+//     rounding vanishes with (u - 1, v - 1) -- the fixed point is the FP64 one.
+// Absorption (FP64, every absorb_every iterations): f += reg log u,
+// g += reg log v (log u, log v are kept in FP32 between absorptions); r0 = alpha .* (K beta), c0 = beta .* (K^T alpha) by FP64
+// MFMA GEMMs; the L1 marginal error of the absorbed plan decides convergence
+// (a target stops at the first absorption with error <= tol) and the form of
+// the next cycle.
+#include "gemm.hpp"
+#include "kernels_fp64.hpp"
+
+namespace rwig {
+
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// per-target L1 (or reference) marginal error of the absorbed plan, stopping
+// decision, form of the next cycle; one block per target
+__global__ void __launch_bounds__(256)
+k_dense_err(int M, int B, const double *__restrict__ a, const double *__restrict__ r0,
+            const int *__restrict__ col_ptr, const double *__restrict__ b,
+            const double *__restrict__ c0, int err_mode, double tol, int iter, int maxiter,
+            double switch_thr, int *done, int *mode, int *iters, double *errs,
+            int *ndone) {
+  __shared__ double sh[256 / kWave];
+  const int t = blockIdx.x;
+  if (done[t]) return;
+  double e1 = 0.0, e2 = 0.0;
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    const double r = r0[i + (long)t * M];
+    if (err_mode == 0) {
+      const double d = log(r) - log(a[i]);
+      e1 += d * d;
+    } else {
+      e1 += fabs(r - a[i]);
+    }
+  }
+  for (int p = col_ptr[t] + threadIdx.x; p < col_ptr[t + 1]; p += blockDim.x) {
+    if (err_mode == 0) {
+      const double d = log(c0[p]) - log(b[p]);
+      e2 += d * d;
+    } else {
+      e2 += fabs(c0[p] - b[p]);
+    }
+  }
+  e1 = block_sum(e1, sh);
+  e2 = block_sum(e2, sh);
+  if (threadIdx.x == 0) {
+    const double err = (err_mode == 0) ? sqrt(e1) + sqrt(e2) : e1 + e2;
+    errs[t] = err;
+    iters[t] = iter;
+    if (!(iter < maxiter && err >= tol)) {
+      done[t] = 1;
+      atomicAdd(ndone, 1);
+    }
+    mode[t] = (err < switch_thr) ? 1 : 0;
+  }
+}
+
+// transport cost and regularised loss of the absorbed plan, one block per target
+__global__ void __launch_bounds__(256)
+k_dense_loss(const double *__restrict__ C, long ldc, int M, const int *__restrict__ col_ptr,
+             const int *__restrict__ col_idx, const double *__restrict__ F,
+             const double *__restrict__ G, double reg, double *loss) {
+  __shared__ double sh[256 / kWave];
+  const int t = blockIdx.x;
+  double cc = 0.0, ce = 0.0;
+  for (int p = col_ptr[t]; p < col_ptr[t + 1]; ++p) {
+    const long col = col_idx[p];
+    const double gq = G[p];
+    for (int i = threadIdx.x; i < M; i += blockDim.x) {
+      const double cij = C[i + col * ldc];
+      const double rr = (cij - F[i + (long)t * M]) - gq;
+      const double pv = exp(-rr / reg);
+      cc += cij * pv;
+      ce += pv * (-rr / reg - 1.0);
+    }
+  }
+  cc = block_sum(cc, sh);
+  ce = block_sum(ce, sh);
+  if (threadIdx.x == 0) {
+    loss[2 * t] = cc;
+    loss[2 * t + 1] = cc + reg * ce;
+  }
+}
+
+template <typename T> T *alloc_into(std::vector<DBuf<char>> &pool, size_t n) {
+  pool.emplace_back(n * sizeof(T));
+  return reinterpret_cast<T *>(pool.back().get());
+}
+
+}  // namespace
+
+// Shared-cost data of the dense path: K in BF16 and FP64, both orientations.
+struct DenseKernel {
+  int M = 0, Nc = 0;
+  long ldm = 0, ldn = 0;  // padded M, padded Nc (multiples of 8)
+  DBuf<uint16_t> Kb, KbT;  // Kb: Kb[i + j*ldm] = K(i,j); KbT: KbT[j + i*ldn] = K(i,j)
+  DBuf<double> Kd, KdT;
+};
+
+void dense_kernel_build(hipStream_t st, const double *C, long ldc, int M, int Nc, double reg,
+                        DenseKernel &dk) {
+  dk.M = M;
+  dk.Nc = Nc;
+  dk.ldm = (M + 7) / 8 * 8;
+  dk.ldn = (Nc + 7) / 8 * 8;
+  dk.Kb.reset((size_t)dk.ldm * Nc);
+  dk.KbT.reset((size_t)dk.ldn * M);
+  dk.Kd.reset((size_t)dk.ldm * Nc);
+  dk.KdT.reset((size_t)dk.ldn * M);
+  uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
+  double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
+  const long ldm = dk.ldm, ldn = dk.ldn;
+  launch_map(st, ldm * Nc, [=] __device__(long k) {
+    const long i = k % ldm, j = k / ldm;
+    const double v = (i < M) ? exp(-C[i + j * ldc] / reg) : 0.0;
+    Kd[k] = v;
+    Kb[k] = f2bf((float)v);
+  });
+  launch_map(st, ldn * M, [=] __device__(long k) {
+    const long j = k % ldn, i = k / ldn;
+    const double v = (j < Nc) ? exp(-C[i + j * ldc] / reg) : 0.0;
+    KdT[k] = v;
+    KbT[k] = f2bf((float)v);
+  });
+}
+
+void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const DenseKernel &dk,
+                            const double *a, int B, const int *col_ptr, const int *col_idx,
+                            const int *tgt_of, int nnz, const double *b, double reg,
+                            int maxiter, double tol, int err_mode, int absorb_every,
+                            double switch_thr, double *F, double *G, int *iters, double *errs,
+                            double *loss) {
+  const int M = dk.M, Nc = dk.Nc;
+  const long ldm = dk.ldm, ldn = dk.ldn;
+  const long MB = (long)M * B;
+  std::vector<DBuf<char>> pool;
+  // per-target state
+  float *lu = alloc_into<float>(pool, MB), *al = alloc_into<float>(pool, MB);
+  float *ra = alloc_into<float>(pool, MB);
+  double *r0 = alloc_into<double>(pool, MB);
+  float *lv = alloc_into<float>(pool, nnz), *be = alloc_into<float>(pool, nnz);
+  float *sb = alloc_into<float>(pool, nnz);
+  double *c0 = alloc_into<double>(pool, nnz);
+  int *pos = alloc_into<int>(pool, (size_t)Nc * B);
+  int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
+  int *ndone = alloc_into<int>(pool, 1);
+  // GEMM operands
+  uint16_t *W = alloc_into<uint16_t>(pool, (size_t)ldm * B);
+  uint16_t *X = alloc_into<uint16_t>(pool, (size_t)ldn * B);
+  double *Ad = alloc_into<double>(pool, (size_t)ldm * B);
+  double *Bd = alloc_into<double>(pool, (size_t)ldn * B);
+  float *Yc = alloc_into<float>(pool, MB);
+
+  const uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
+  const double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
+
+  // ---- initial state
+  launch_map(st, MB, [=] __device__(long k) {
+    F[k] = 0.0;
+    lu[k] = 0.0f;
+  });
+  launch_map(st, (long)nnz, [=] __device__(long p) {
+    G[p] = 0.0;
+    lv[p] = 0.0f;
+  });
+  launch_map(st, (long)Nc * B, [=] __device__(long k) { pos[k] = -1; });
+  launch_map(st, (long)nnz, [=] __device__(long p) {
+    pos[col_idx[p] + (long)tgt_of[p] * Nc] = (int)p;
+  });
+  launch_map(st, (long)ldm * B, [=] __device__(long k) {
+    W[k] = 0;
+    Ad[k] = 0.0;
+  });
+  launch_map(st, (long)ldn * B, [=] __device__(long k) {
+    X[k] = 0;
+    Bd[k] = 0.0;
+  });
+  launch_map(st, (long)B, [=] __device__(long t) {
+    done[t] = 0;
+    mode[t] = 0;
+  });
+  launch_map(st, 1L, [=] __device__(long) { *ndone = 0; });
+
+  auto absorb = [&](int iter) {
+    // fold the scalings into the potentials (targets still running)
+    launch_map(st, MB, [=] __device__(long k) {
+      const long t = k / M, i = k % M;
+      if (!done[t]) {
+        const double fk = F[k] + reg * (double)lu[k];
+        F[k] = fk;
+        const double ak = exp(fk / reg);
+        al[k] = (float)ak;
+        Ad[i + t * ldm] = ak;
+      }
+    });
+    launch_map(st, (long)nnz, [=] __device__(long p) {
+      const long t = tgt_of[p];
+      if (!done[t]) {
+        const double gq = G[p] + reg * (double)lv[p];
+        G[p] = gq;
+        const double bq = exp(gq / reg);
+        be[p] = (float)bq;
+        Bd[col_idx[p] + t * ldn] = bq;
+      }
+    });
+    // exact marginals of the absorbed plan (FP64 MFMA)
+    gemm_tn<double>(st, M, B, Nc, KdT, ldn, Bd, ldn,
+                    [=] __device__(int i, int t, double v) {
+                      if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
+                    });
+    gemm_tn<double>(st, Nc, B, M, Kd, ldm, Ad, ldm, [=] __device__(int j, int t, double v) {
+      const int p = pos[j + (long)t * Nc];
+      if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
+    });
+    hipLaunchKernelGGL(k_dense_err, dim3(B), dim3(256), 0, st, M, B, a, r0, col_ptr, b, c0,
+                       err_mode, tol, iter, maxiter, switch_thr, done, mode, iters, errs,
+                       ndone);
+    check_launch();
+    // calibration of the multiplicative form: the same BF16 products at u = v = 1
+    launch_map(st, MB, [=] __device__(long k) {
+      const long t = k / M, i = k % M;
+      W[i + t * ldm] = f2bf(al[k]);
+    });
+    launch_map(st, (long)nnz, [=] __device__(long p) {
+      X[col_idx[p] + (long)tgt_of[p] * ldn] = f2bf(be[p]);
+    });
+    gemm_tn<uint16_t>(st, M, B, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
+      Yc[i + (long)t * M] = y;
+    });
+    gemm_tn<uint16_t>(st, Nc, B, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
+      const int p = pos[j + (long)t * Nc];
+      if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
+    });
+    // u = a / r0 with v = 1; W for the first product of the cycle
+    launch_map(st, MB, [=] __device__(long k) {
+      const long t = k / M, i = k % M;
+      if (done[t]) return;
+      ra[k] = (float)(r0[k] / (double)Yc[k]);
+      const double u = a[i] / r0[k];
+      lu[k] = (float)log(u);
+      W[i + t * ldm] = f2bf(mode[t] ? (float)((double)al[k] * (u - 1.0)) : (float)((double)al[k] * u));
+    });
+    launch_map(st, (long)nnz, [=] __device__(long p) {
+      if (!done[tgt_of[p]]) lv[p] = 0.0f;
+    });
+  };
+
+  auto iterate = [&]() {
+    // Z = K^T W -> v, X
+    gemm_tn<uint16_t>(st, Nc, B, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
+      const int p = pos[j + (long)t * Nc];
+      if (p < 0 || done[t]) return;
+      double v;
+      if (mode[t]) {
+        v = b[p] / (c0[p] + (double)be[p] * (double)z);
+        X[j + (long)t * ldn] = f2bf((float)((double)be[p] * (v - 1.0)));
+      } else {
+        v = b[p] / ((double)sb[p] * (double)z);
+        X[j + (long)t * ldn] = f2bf((float)((double)be[p] * v));
+      }
+      lv[p] = (float)log(v);
+    });
+    // Y = K X -> u, W
+    gemm_tn<uint16_t>(st, M, B, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
+      if (done[t]) return;
+      const long k = i + (long)t * M;
+      double u;
+      if (mode[t]) {
+        u = a[i] / (r0[k] + (double)al[k] * (double)y);
+        W[i + (long)t * ldm] = f2bf((float)((double)al[k] * (u - 1.0)));
+      } else {
+        u = a[i] / ((double)ra[k] * (double)y);
+        W[i + (long)t * ldm] = f2bf((float)((double)al[k] * u));
+      }
+      lu[k] = (float)log(u);
+    });
+  };
+
+  int iter = 0, hdone = 0;
+  absorb(iter);
+  while (true) {
+    d2h(&hdone, ndone, 1, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    if (hdone >= B || iter >= maxiter) break;
+    const int n = std::min(absorb_every, maxiter - iter);
+    for (int s = 0; s < n; ++s) iterate();
+    iter += n;
+    absorb(iter);
+  }
+  if (loss) {
+    hipLaunchKernelGGL(k_dense_loss, dim3(B), dim3(256), 0, st, C, ldc, M, col_ptr, col_idx, F,
+                       G, reg, loss);
+    check_launch();
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace rwig
+
+using namespace rwig;
+
+extern "C" {
+
+int rwig_dense_kernel_create(const double *C, long ldc, int M, int Nc, double reg, void *stream,
+                             void **handle) {
+  return guarded("rwig_dense_kernel_create", [&] {
+    RWIG_REQUIRE(C && handle && M > 0 && Nc > 0 && reg > 0.0, "invalid argument");
+    auto *dk = new DenseKernel();
+    try {
+      dense_kernel_build((hipStream_t)stream, C, ldc, M, Nc, reg, *dk);
+      RWIG_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    } catch (...) {
+      delete dk;
+      throw;
+    }
+    *handle = dk;
+  });
+}
+
+int rwig_dense_kernel_destroy(void *handle) {
+  return guarded("rwig_dense_kernel_destroy", [&] {
+    RWIG_HIP_CHECK(hipDeviceSynchronize());
+    delete static_cast<DenseKernel *>(handle);
+  });
+}
+
+int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc, const double *a,
+                                   int B, const int *col_ptr, const int *col_idx,
+                                   const int *tgt_of, int nnz, const double *b, double reg,
+                                   int max_iter, double zero_tol, int err_mode,
+                                   int absorb_every, double switch_thr, double *f, double *g,
+                                   int *iters, double *errs, double *loss, void *stream) {
+  return guarded("rwig_sinkhorn_batched_mfma_dev", [&] {
+    RWIG_REQUIRE(handle != nullptr, "null kernel handle");
+    RWIG_REQUIRE(B >= 1 && nnz >= B, "every target needs a support");
+    RWIG_REQUIRE(reg > 0.0 && max_iter >= 0 && absorb_every >= 1, "invalid argument");
+    RWIG_REQUIRE(err_mode == 0 || err_mode == 1, "err_mode must be 0 or 1");
+    RWIG_REQUIRE(C && a && col_ptr && col_idx && tgt_of && b && f && g && iters && errs,
+                 "null argument");
+    const DenseKernel &dk = *static_cast<const DenseKernel *>(handle);
+    sinkhorn_batched_dense((hipStream_t)stream, C, ldc, dk, a, B, col_ptr, col_idx, tgt_of, nnz,
+                           b, reg, max_iter, zero_tol, err_mode, absorb_every, switch_thr, f, g,
+                           iters, errs, loss);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+// MFMA GEMM for gfx950 (CDNA4):  D(m, n) = sum_k A(m, k) B(k, n)
+//
+// Both operands are read along k: A is row-major (A(m, k) = A[m * lda + k],
+// i.e. a column-major matrix used transposed) and B is column-major
+// (B(k, n) = B[k + n * ldb]). The Gibbs products of the batched solvers are of
+// that form once the kernel is kept in both orientations: K v uses K^T stored
+// column-major as the row-major K, and K^T u uses K stored column-major.
+//
+// Workgroup tile BM x BN over k-steps of BK, staged through LDS in k-major
+// rows (each row of BK contiguous k values, padded against bank conflicts);
+// 4 wavefronts in a 2 x 2 grid, each computing a (BM/2) x (BN/2) block with
+// 16 x 16 MFMA tiles. The next k-step is loaded into registers while the
+// current one is multiplied. D is written through an epilogue functor
+// op(m, n, value) so callers can fuse their elementwise update.
+#pragma once
+
+#include "common.hpp"
+
+namespace rwig {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+
+// One MFMA flavour: element type, k per instruction, fragment of one lane.
+//   bf16: v_mfma_f32_16x16x32_bf16, A/B fragments of 8 consecutive k
+//   fp64: v_mfma_f64_16x16x4_f64,   A/B fragments of 1 k
+// Lane l of a 16 x 16 tile: A row l % 16, B column l % 16, k block l / 16;
+// accumulator register r < 4 of lane l: column l % 16 and row
+//   bf16: 4 (l / 16) + r        fp64: 4 r + l / 16
+// (the f64 layout is the one composable_kernel's xdlops tables give:
+// group size 1, four input blocks).
+template <typename T> struct Mfma;
+
+template <> struct Mfma<uint16_t> {
+  using elem = uint16_t;
+  using frag_t = bf16x8_t;
+  using acc_t = floatx4_t;
+  using out_t = float;
+  static constexpr int KI = 32;  // k per instruction
+  static constexpr int KL = 8;   // k per lane
+  __device__ static acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static frag_t load(const elem *p) { return *reinterpret_cast<const frag_t *>(p); }
+  __device__ static void mma(acc_t &c, const frag_t &a, const frag_t &b) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <> struct Mfma<double> {
+  using elem = double;
+  using frag_t = double;
+  using acc_t = doublex4_t;
+  using out_t = double;
+  static constexpr int KI = 4;
+  static constexpr int KL = 1;
+  __device__ static acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static frag_t load(const elem *p) { return *p; }
+  __device__ static void mma(acc_t &c, const frag_t &a, const frag_t &b) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<uint16_t> {
+  static constexpr int BM = 128, BN = 128, BK = 64, PAD = 8;
+};
+template <> struct GemmCfg<double> {
+  static constexpr int BM = 64, BN = 64, BK = 16, PAD = 2;
+};
+
+// Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix
+// stored with its k values contiguous (element (r, k) at p[r * ld + k]) into
+// registers; rows >= nrows and k >= K read as zero. Vector width: 16 bytes.
+template <typename T, int ROWS, int BK, int NT>
+struct TileLoader {
+  static constexpr int VEC = 16 / sizeof(T);
+  static constexpr int CHUNKS = ROWS * BK / VEC;      // 16-byte chunks per tile
+  static constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
+  uint4 reg[PER_THREAD];
+
+  __device__ void load(const T *__restrict__ p, long ld, int nrows, int K, int r0, int k0) {
+#pragma unroll
+    for (int c = 0; c < PER_THREAD; ++c) {
+      const int idx = threadIdx.x + c * NT;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (idx < CHUNKS) {
+        const int row = idx / (BK / VEC), kc = (idx % (BK / VEC)) * VEC;
+        const int r = r0 + row, k = k0 + kc;
+        if (r < nrows) {
+          const T *src = p + (long)r * ld + k;
+          if (k + VEC <= K) {
+            v = *reinterpret_cast<const uint4 *>(src);
+          } else {
+            T tmp[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) tmp[e] = (k + e < K) ? src[e] : T(0);
+            v = *reinterpret_cast<const uint4 *>(tmp);
+          }
+        }
+      }
+      reg[c] = v;
+    }
+  }
+  __device__ void store(T *lds, int ldl) const {
+#pragma unroll
+    for (int c = 0; c < PER_THREAD; ++c) {
+      const int idx = threadIdx.x + c * NT;
+      if (idx < CHUNKS) {
+        const int row = idx / (BK / VEC), kc = (idx % (BK / VEC)) * VEC;
+        *reinterpret_cast<uint4 *>(lds + row * ldl + kc) = reg[c];
+      }
+    }
+  }
+};
+
+template <typename T, class Op>
+__global__ void __launch_bounds__(256)
+k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
+          long ldb, Op op) {
+  using MF = Mfma<T>;
+  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN, BK = GemmCfg<T>::BK;
+  constexpr int LDL = BK + GemmCfg<T>::PAD;
+  constexpr int WM = BM / 2, WN = BN / 2;  // per-wave block
+  constexpr int TM = WM / 16, TN = WN / 16;
+  __shared__ __attribute__((aligned(16))) T As[BM * LDL];
+  __shared__ __attribute__((aligned(16))) T Bs[BN * LDL];
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv & 1, wn = wv >> 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+
+  typename MF::acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
+
+  TileLoader<T, BM, BK, 256> la;
+  TileLoader<T, BN, BK, 256> lb;
+  la.load(A, lda, M, K, m0, 0);
+  lb.load(B, ldb, N, K, n0, 0);
+  const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+    la.store(As, LDL);
+    lb.store(Bs, LDL);
+    __syncthreads();
+    if (k0 + BK < K) {
+      la.load(A, lda, M, K, m0, k0 + BK);
+      lb.load(B, ldb, N, K, n0, k0 + BK);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += MF::KI) {
+      typename MF::frag_t af[TM], bfr[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = MF::load(As + (wm * WM + i * 16 + lr) * LDL + kk + lk);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bfr[j] = MF::load(Bs + (wn * WN + j * 16 + lr) * LDL + kk + lk);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
+    }
+  }
+  // accumulator lane layout: column lr, row MF::row(lane, r)
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WN + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WM + i * 16 + MF::row(lane, r);
+        if (m < M && n < N) op(m, n, acc[i][j][r]);
+      }
+    }
+}
+
+template <typename T, class Op>
+void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
+             Op op) {
+  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
+  RWIG_REQUIRE(M > 0 && N > 0 && K > 0, "GEMM dimensions must be positive");
+  RWIG_REQUIRE(lda % (16 / sizeof(T)) == 0 && ldb % (16 / sizeof(T)) == 0,
+               "GEMM leading dimensions must be multiples of 16 bytes");
+  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN);
+  hipLaunchKernelGGL((k_gemm_tn<T, Op>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, op);
+  check_launch();
+}
+
+}  // namespace rwig

@@ -128,3 +128,37 @@ def test_c2_scaled_down_fp64_against_oracle():
         o = O.sinkhorn_log(a, bfull, C, reg)
         assert it[t] == o["iter"]
         assert np.abs(f[t] - o["f"]).max() / np.abs(o["f"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("absorb_every", [10, 20])
+def test_mfma_path_reaches_tolerance_and_matches_fp64(absorb_every):
+    C, a, sup, wts, _ = make(21, 1024, 9, 30, 70, 0.01)
+    reg, tol = 0.01, 1e-6
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    ref = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64", err="l1",
+                                   zero_tol=tol, max_iter=5000), tb)
+    out = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, zero_tol=tol, max_iter=5000,
+                                        absorb_every=absorb_every), tb)
+    for t in range(tb.B):
+        e_ref, c_ref = plan_l1(C, a, sup[t], wts[t], ref[0][t], ref[1][t], reg)
+        e, c = plan_l1(C, a, sup[t], wts[t], out[0][t], out[1][t], reg)
+        print("mfma", t, "iters", out[2][t], "vs", ref[2][t], "err", e, flush=True)
+        assert e <= tol * 1.0001, (t, e)
+        assert abs(out[3][t] - e) <= 1e-9 + 1e-6 * e
+        assert abs(c - c_ref) <= 1e-5 * abs(c_ref)
+        assert abs(out[4][t, 0] - c) <= 1e-9 * abs(c)
+
+
+def test_mfma_path_alone_equals_in_mix():
+    C, a, sup, wts, reg = make(23, 600, 6, 10, 60, 0.02)
+    Cd, ad = to_dev(C, a)
+    kern = RB.DenseKernel(Cd, reg)
+    tb = RB.TargetBatch(sup, wts)
+    mix = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern), tb)
+    for t in [0, 5]:
+        tb1 = RB.TargetBatch([sup[t]], [wts[t]])
+        one = host(RB.sinkhorn_batched_mfma(Cd, ad, tb1, reg, kernel=kern), tb1)
+        assert np.array_equal(one[0][0], mix[0][t])
+        assert np.array_equal(one[1][0], mix[1][t])
+        assert one[2][0] == mix[2][t]

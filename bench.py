@@ -8,10 +8,11 @@ eps = 0.01, every solve run until its L1 marginal error
 sum|P1 - a| + sum|P^T 1 - b| (FP64, on the GPU) is <= 1e-6.
 
 A step is one batched solve of all 1024 targets with inputs resident in HBM.
-The headline value is BF16 solves/s (BF16 Gibbs kernel, FP32 iterations, FP64
-absorption every 20 iterations); FP32, the batched FP64 log-domain form and
-the plain FP64 per-call form (one rwig_sinkhorn_log_dev call per target) are
-measured beside it, and the CPU oracle (the reference algorithm restated in
+The headline value is BF16 solves/s of the MFMA path (every Gibbs product of
+an iteration one BF16 MFMA GEMM over all targets, FP32 accumulation, FP64
+absorption every 20 iterations); the resident per-target kernels (BF16, FP32),
+the batched FP64 log-domain form and the plain FP64 per-call form (one
+rwig_sinkhorn_log_dev call per target) are measured beside it, and the CPU oracle (the reference algorithm restated in
 C++) is timed on a bounded sample of targets on the host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -32,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TF = 2500.0
 
 
 def log(msg):
@@ -85,64 +87,71 @@ def main():
             f"max err {err.max():.3e}, all converged {ok}")
         return ok, it, err
 
-    # ---------------------------------------------------------------- BF16 headline
-    log("building the BF16 Gibbs kernel")
-    Klo = RB.gibbs_lowp(Cd, reg, "bf16")
-    out = run("bf16", Klo)
+    # ---------------------------------------------------------------- BF16 MFMA headline
+    log("building the Gibbs kernels (BF16 and FP64, both orientations)")
+    kern = RB.DenseKernel(Cd, reg)
+
+    def run_mfma(out=None):
+        return RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, zero_tol=tol,
+                                        max_iter=1000, absorb_every=20, out=out)
+
+    out = run_mfma()
     for _ in range(args.warmup):
-        run("bf16", Klo, out)
+        run_mfma(out)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(args.steps)]
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         ev[s][0].record(stream)
-        run("bf16", Klo, out)
+        run_mfma(out)
         ev[s][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
-    ok_bf16, it_bf16, _ = check_out(out, "bf16")
+    ok_mfma, it_mfma, _ = check_out(out, "bf16-mfma")
     value = B * args.steps / wall
-    kern_ms = float(step_ms.mean())
 
-    # algorithmic bytes of one launch of the resident BF16 kernel: per target,
-    # every iteration streams its nt columns of the BF16 kernel (M x nt x 2 B)
-    # and every absorption its nt cost columns in FP64 (M x nt x 8 B)
-    n_abs = np.ceil(it_bf16 / 20.0) + 1
-    alg_bytes = float(np.sum(it_bf16 * nts * M * 2.0 + n_abs * nts * M * 8.0))
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    # dominant kernel: the BF16 MFMA GEMM K^T W (M x B x M); its average launch
+    # time measured here with HIP events on the same stream, same shape
+    gM, gN, gK = M, B, M
+    Wt = torch.randn((gN, (gK + 7) // 8 * 8), device=dev).to(torch.bfloat16)
+    Kt = torch.randn((gM, (gK + 7) // 8 * 8), device=dev).to(torch.bfloat16)
+    Dt = torch.empty((gN, gM), dtype=torch.float32, device=dev)
 
-    # ---------------------------------------------------------------- FP32 and FP64 batched
+    def gemm():
+        check(lib().rwig_gemm_tn_bf16_dev(gM, gN, gK, Kt.data_ptr(), Kt.shape[1], Wt.data_ptr(),
+                                          Wt.shape[1], Dt.data_ptr(), gM, stream.cuda_stream))
+
+    gemm()
+    torch.cuda.synchronize()
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    g0.record(stream)
+    for _ in range(reps):
+        gemm()
+    g1.record(stream)
+    torch.cuda.synchronize()
+    gemm_ms = g0.elapsed_time(g1) / reps
+    gemm_flops = 2.0 * gM * gN * gK
+    achieved_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
+
+    # ---------------------------------------------------------------- other forms
     def timed(precision, Klo=None, reps=1):
-        o = run(precision, Klo)
+        o = RB.sinkhorn_batched(Cd, ad, tb, reg, precision=precision, max_iter=1000,
+                                zero_tol=tol, err="l1", absorb_every=20, Klo=Klo)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(reps):
-            run(precision, Klo, o)
+            RB.sinkhorn_batched(Cd, ad, tb, reg, precision=precision, max_iter=1000,
+                                zero_tol=tol, err="l1", absorb_every=20, Klo=Klo, out=o)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / reps
-        ok, it, _ = check_out(o, precision)
+        ok, it, _ = check_out(o, precision + "-resident")
         return B / dt, ok, it
 
-    # MFMA path: all targets as GEMM columns (BF16 products, FP64 absorption)
-    log("MFMA path")
-    kern = RB.DenseKernel(Cd, reg)
-    o = RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, zero_tol=tol, max_iter=1000,
-                                 absorb_every=20)
-    torch.cuda.synchronize()
-    tm = time.perf_counter()
-    for _ in range(2):
-        RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, zero_tol=tol, max_iter=1000,
-                                 absorb_every=20, out=o)
-    torch.cuda.synchronize()
-    mfma_sps = B / ((time.perf_counter() - tm) / 2)
-    okm, itm, _ = check_out(o, "bf16-mfma")
-    log(f"bf16-mfma: {mfma_sps:.1f} solves/s")
-
-    Klo32 = RB.gibbs_lowp(Cd, reg, "fp32")
-    fp32_sps, ok32, it32 = timed("fp32", Klo32, reps=2)
+    bf16r_sps, okr, itr = timed("bf16", RB.gibbs_lowp(Cd, reg, "bf16"), reps=2)
+    fp32_sps, ok32, it32 = timed("fp32", RB.gibbs_lowp(Cd, reg, "fp32"), reps=2)
     fp64_sps, ok64, it64 = timed("fp64", reps=1)
 
     # ---------------------------------------------------------------- plain FP64 per call
@@ -199,6 +208,8 @@ def main():
         "ms_per_step": wall / args.steps * 1e3,
         "p50_ms": float(np.percentile(step_ms, 50)),
         "p99_ms": float(np.percentile(step_ms, 99)),
+        "latency_note": "p50/p99 of the batch completion time (every target of a step "
+                        "completes within it)",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -208,19 +219,24 @@ def main():
             "workload": "batched log-stabilised Sinkhorn, one source vs many targets",
             "n": M, "targets": B, "support": int(nts.max()), "eps": reg,
             "stop": "L1 marginal error <= 1e-6 (FP64 on GPU)",
-            "precision": "BF16 Gibbs kernel, FP32 iterations, FP64 absorption every 20",
+            "precision": "BF16 MFMA products (FP32 accumulation), FP64 absorption every 20 "
+                         "iterations (FP64 MFMA marginals)",
         },
-        "converged": {"bf16": ok_bf16, "bf16_mfma": okm, "fp32": ok32, "fp64": ok64},
-        "iterations_mean": {"bf16": float(it_bf16.mean()), "fp32": float(it32.mean()),
-                            "fp64": float(it64.mean())},
-        "modes_solves_per_s": {"bf16": value, "bf16_mfma": mfma_sps, "fp32": fp32_sps,
-                               "fp64_batched": fp64_sps, "fp64_plain_per_call": plain_sps},
+        "converged": {"bf16_mfma": ok_mfma, "bf16_resident": okr, "fp32_resident": ok32,
+                      "fp64_batched": ok64},
+        "iterations_mean": {"bf16_mfma": float(it_mfma.mean()),
+                            "bf16_resident": float(itr.mean()),
+                            "fp32_resident": float(it32.mean()),
+                            "fp64_batched": float(it64.mean())},
+        "modes_solves_per_s": {"bf16_mfma": value, "bf16_resident": bf16r_sps,
+                               "fp32_resident": fp32_sps, "fp64_batched": fp64_sps,
+                               "fp64_plain_per_call": plain_sps},
         "speedup_vs_plain_fp64": value / plain_sps,
         "speedup_vs_fp64_batched": value / fp64_sps,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sinkhorn_fast_resident<bf16>", "kernel_ms": kern_ms,
-                     "alg_bytes_per_launch": alg_bytes},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": MFMA_BF16_PEAK_TF,
+                     "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TF,
+                     "traffic": None, "kernel": "k_gemm_tn<bf16> (K^T W, 4096 x 1024 x 4096)",
+                     "kernel_ms": gemm_ms, "alg_flops_per_launch": gemm_flops},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -24,10 +24,11 @@ typedef double doublex4_t __attribute__((ext_vector_type(4)));
 
 // One MFMA flavour: element type, k per instruction, fragment of one lane.
 //   bf16: v_mfma_f32_16x16x32_bf16, A/B fragments of 8 consecutive k
+//   fp32: v_mfma_f32_16x16x4_f32,   A/B fragments of 1 k
 //   fp64: v_mfma_f64_16x16x4_f64,   A/B fragments of 1 k
 // Lane l of a 16 x 16 tile: A row l % 16, B column l % 16, k block l / 16;
 // accumulator register r < 4 of lane l: column l % 16 and row
-//   bf16: 4 (l / 16) + r        fp64: 4 r + l / 16
+//   bf16, fp32: 4 (l / 16) + r        fp64: 4 r + l / 16
 // (the f64 layout is the one composable_kernel's xdlops tables give:
 // group size 1, four input blocks).
 template <typename T> struct Mfma;
@@ -62,7 +63,25 @@ template <> struct Mfma<double> {
   __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 
+template <> struct Mfma<float> {
+  using elem = float;
+  using frag_t = float;
+  using acc_t = floatx4_t;
+  using out_t = float;
+  static constexpr int KI = 4;
+  static constexpr int KL = 1;
+  __device__ static acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static frag_t load(const elem *p) { return *p; }
+  __device__ static void mma(acc_t &c, const frag_t &a, const frag_t &b) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 template <typename T> struct GemmCfg;
+template <> struct GemmCfg<float> {
+  static constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
+};
 template <> struct GemmCfg<uint16_t> {
   static constexpr int BM = 128, BN = 128, BK = 64, PAD = 8;
 };

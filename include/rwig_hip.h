@@ -93,6 +93,43 @@ int rwig_sinkhorn_log_dev(const double *a, const double *b, const double *C,
                           double *g, double *loss, int *iter_out,
                           double *err_out, void *stream);
 
+/* Wasserstein Dictionary Learning. Replaces cuda_wdl
+ * (inst/include/cuda_interface.cuh:31-39, src/cuda_wdl.cu:539-818), same
+ * argument order: Y (N x M documents), C (N x N cost); A (N x S topics),
+ * W (S x M weights) and Yhat (N x M barycenters) are filled. Alpha, Lambda
+ * start from std::mt19937(seed) standard normals (Alpha first), as in
+ * src/cuda_wdl.cu:592-593, :677-689. The training step is
+ * WassersteinDictionaryLearning::_train_batch_batched
+ * (src/wdl_impl.cpp:87-247) with the reference's batch-size and g_Lambda
+ * broadcast rules; Yhat comes from the parallel barycenter, batched over
+ * documents, each stopping at its own iteration. FP64. */
+int rwig_wdl(double *A, double *W, double *Yhat, const double *Y,
+             const double *C, int N, int M, int S, double reg, int max_iter,
+             double zero_tol, int B, int E, int opt, double eta, double gamma,
+             double beta1, double beta2, double eps, int seed, int verbose);
+
+/* rwig_wdl from explicit initial Alpha0 (N x S) and Lambda0 (S x M) -- the
+ * values the reference's CPU path draws with R's RNG
+ * (src/wdl_impl.cpp:277-278). precision 0: FP64, 1: FP32 Gibbs products and
+ * recurrences (optimizer, softmax in FP64). */
+int rwig_wdl_init(double *A, double *W, double *Yhat, const double *Y,
+                  const double *C, int N, int M, int S, double reg,
+                  int max_iter, double zero_tol, int B, int E, int opt,
+                  double eta, double gamma, double beta1, double beta2,
+                  double eps, const double *Alpha0, const double *Lambda0,
+                  int precision);
+
+/* One batched WDL gradient on device pointers (src/wdl_impl.cpp:87-247):
+ * K = exp(-C/reg) and KT its transpose, column-major, leading dimension ldk
+ * (>= N, multiple of 8); A (N x S), Yb (N x D), Wb (S x D) in the precision
+ * given (0 double, 1 float); outputs FP64: gAlpha (N x S), glambda (S) and
+ * loss = sum ||b_L - y||^2 over the batch (may be null). */
+int rwig_wdl_batch_grad_dev(const void *K, const void *KT, long ldk, int N,
+                            int S, int D, int L, const void *A,
+                            const void *Yb, const void *Wb, double *gAlpha,
+                            double *glambda, double *loss, int precision,
+                            void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batched solves, device pointers (no counterpart in the reference, whose GPU
  * path solves one problem per call, inst/include/cuda_interface.cuh:13-39; the

@@ -161,3 +161,54 @@ def barycenter(A, C, w, b_ext=None, barycenter_control=None):
 def cuda_available():
     """cuda_available_cpp() (src/check_cuda.cpp:6-12): is a HIP device usable."""
     return bool(lib().rwig_available())
+
+
+_OPTIMIZERS = {"sgd": 0, "adam": 1, "adamw": 2}
+
+
+def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, usecuda=True,
+        max_iter=20, zero_tol=1e-6, optimizer=2, eta=0.001, gamma=0.01, beta1=0.9,
+        beta2=0.999, eps=1e-8, verbose=False, seed=42, init=None, precision="fp64"):
+    """Wasserstein Dictionary Learning with the argument contract of wdl_cpp
+    (src/wdl.cpp:108-170): Y (N x M) normalised documents, C (N x N) cost.
+
+    Returns dict(A=N x S topics, W=S x M weights, Yhat=N x M barycenters).
+    With init=(Alpha0, Lambda0) the explicit starting parameters replace the
+    seeded draw (the reference's CPU path draws them with R's RNG);
+    precision "fp32" runs the Gibbs products and recurrences in FP32.
+    Only the parallel barycenter (sinkhorn_mode 1) is on the GPU path."""
+    if sinkhorn_mode not in (1, 2):
+        raise ValueError("Sinkhorn mode not supported")
+    if isinstance(optimizer, str):
+        optimizer = _OPTIMIZERS[optimizer]
+    if optimizer not in (0, 1, 2):
+        raise ValueError("optimizer must be: 0, 1, 2!")
+    Y = _f64(Y)
+    C = _f64(C)
+    N, M = Y.shape
+    if N <= S:
+        raise ValueError("Number of topics S must be smaller than the vocab size N!")
+    if sinkhorn_mode != 1:
+        raise ValueError("the GPU WDL path infers Yhat with the parallel barycenter "
+                         "(sinkhorn_mode = 1)")
+    if not usecuda:
+        raise ValueError("this build runs every solve on the GPU; usecuda=False is not offered")
+    A = np.zeros((N, S), order="F")
+    W = np.zeros((S, M), order="F")
+    Yhat = np.zeros((N, M), order="F")
+    args = (float(reg), int(max_iter), float(zero_tol), int(batch_size), int(epochs),
+            int(optimizer), float(eta), float(gamma), float(beta1), float(beta2), float(eps))
+    if init is None:
+        if precision != "fp64":
+            raise ValueError("the seeded entry point runs in FP64")
+        check(lib().rwig_wdl(_p(A), _p(W), _p(Yhat), _p(Y), _p(C), N, M, int(S), *args,
+                             int(seed), int(bool(verbose))))
+    else:
+        Alpha0 = _f64(init[0])
+        Lambda0 = _f64(init[1])
+        if Alpha0.shape != (N, S) or Lambda0.shape != (S, M):
+            raise ValueError("init must be (Alpha0 N x S, Lambda0 S x M)")
+        prec = {"fp64": 0, "fp32": 1}[precision]
+        check(lib().rwig_wdl_init(_p(A), _p(W), _p(Yhat), _p(Y), _p(C), N, M, int(S), *args,
+                                  _p(Alpha0), _p(Lambda0), prec))
+    return {"A": A, "W": W, "Yhat": Yhat}

@@ -1,0 +1,97 @@
+"""GPU tests of Wasserstein Dictionary Learning (rwig_wdl*, librwig_hip.so)
+against the FP64 oracle (src/wdl_impl.cpp restated).
+
+FP64: batched gradient and the whole fit (A, W, Yhat) to 1e-9 relative.
+FP32: batched gradient to 1e-4 relative (FP32 Gibbs products, stated
+tolerance)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+import rwig_amd  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from rwig_amd._lib import check, lib  # noqa: E402
+
+
+def wdl_problem(seed, N, M, S):
+    rng = np.random.default_rng(seed)
+    E = rng.standard_normal((N, 5))
+    E /= np.linalg.norm(E, axis=1, keepdims=True)
+    C = ((E[:, None] - E[None]) ** 2).sum(-1)
+    Y = rng.dirichlet(np.ones(N), size=M).T.copy()
+    return np.asfortranarray(C), np.asfortranarray(Y), rng
+
+
+def rel(x, y):
+    return float(np.max(np.abs(np.asarray(x) - np.asarray(y))) / np.max(np.abs(y)))
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-4)])
+def test_batch_grad_matches_oracle(precision, tol):
+    N, S, D, L, reg = 70, 4, 6, 12, 0.3
+    C, Y, rng = wdl_problem(1, N, D, S)
+    K = np.exp(-C / reg)
+    A = O.softmax_cols(rng.standard_normal((N, S)))
+    Wb = O.softmax_cols(rng.standard_normal((S, D)))
+    gA_o, gl_o, bB_o = O.wdl_batch_grad(K, A, Y, Wb, L)
+    dt = torch.float64 if precision == 0 else torch.float32
+    ldk = (N + 7) // 8 * 8
+    Kt = torch.zeros((N, ldk), dtype=dt)
+    Kt[:, :N] = torch.from_numpy(K.T.copy())  # column j of K as row j: column-major K
+    KTt = torch.zeros((N, ldk), dtype=dt)
+    KTt[:, :N] = torch.from_numpy(K.copy())  # column-major K^T
+    dev = [x.cuda() for x in (Kt, KTt)]
+    At = torch.from_numpy(A.T.copy()).to(dt).cuda()   # column-major N x S
+    Yt = torch.from_numpy(Y.T.copy()).to(dt).cuda()
+    Wt = torch.from_numpy(Wb.T.copy()).to(dt).cuda()
+    gA = torch.zeros((S, N), dtype=torch.float64, device="cuda")
+    gl = torch.zeros(S, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    check(lib().rwig_wdl_batch_grad_dev(dev[0].data_ptr(), dev[1].data_ptr(), ldk, N, S, D, L,
+                                        At.data_ptr(), Yt.data_ptr(), Wt.data_ptr(),
+                                        gA.data_ptr(), gl.data_ptr(), loss.data_ptr(),
+                                        precision, torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert rel(gA.cpu().numpy().T, gA_o) < tol
+    assert rel(gl.cpu().numpy(), gl_o) < tol
+    assert abs(loss.item() - np.sum((bB_o - Y) ** 2)) < tol * np.sum((bB_o - Y) ** 2)
+
+
+def test_wdl_fit_matches_oracle_fp64():
+    # M % batch_size != 0 exercises the reference's last-batch rule
+    N, M, S = 60, 12, 3
+    C, Y, rng = wdl_problem(2, N, M, S)
+    Alpha0 = rng.standard_normal((N, S))
+    Lambda0 = rng.standard_normal((S, M))
+    kw = dict(batch_size=5, epochs=2, max_iter=10, optimizer=2, eta=0.01)
+    r = rwig_amd.wdl(Y, C, 0.3, S, init=(Alpha0, Lambda0), **kw)
+    o = O.wdl(Y, C, 0.3, S, Alpha0, Lambda0, **kw)
+    for k in ["A", "W", "Yhat"]:
+        assert rel(r[k], o[k]) < 1e-9, k
+
+
+@pytest.mark.parametrize("opt", [0, 1])
+def test_wdl_fit_other_optimizers(opt):
+    N, M, S = 40, 8, 2
+    C, Y, rng = wdl_problem(3, N, M, S)
+    Alpha0 = rng.standard_normal((N, S))
+    Lambda0 = rng.standard_normal((S, M))
+    kw = dict(batch_size=4, epochs=1, max_iter=8, optimizer=opt, eta=0.05)
+    r = rwig_amd.wdl(Y, C, 0.5, S, init=(Alpha0, Lambda0), **kw)
+    o = O.wdl(Y, C, 0.5, S, Alpha0, Lambda0, **kw)
+    for k in ["A", "W", "Yhat"]:
+        assert rel(r[k], o[k]) < 1e-9, k
+
+
+def test_wdl_seeded_entry_point():
+    N, M, S = 30, 6, 2
+    C, Y, _ = wdl_problem(4, N, M, S)
+    r1 = rwig_amd.wdl(Y, C, 0.4, S, batch_size=3, epochs=1, max_iter=5, seed=7)
+    r2 = rwig_amd.wdl(Y, C, 0.4, S, batch_size=3, epochs=1, max_iter=5, seed=7)
+    for k in ["A", "W", "Yhat"]:
+        assert np.array_equal(r1[k], r2[k])
+    np.testing.assert_allclose(r1["A"].sum(0), 1, rtol=1e-12)
+    np.testing.assert_allclose(r1["W"].sum(0), 1, rtol=1e-12)
+    np.testing.assert_allclose(r1["Yhat"].sum(0), 1, rtol=1e-12)

@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TF = 2500.0
+ABSORB_MFMA = 30  # iterations between FP64 absorptions on the MFMA path
 
 
 def log(msg):
@@ -93,7 +94,7 @@ def main():
 
     def run_mfma(out=None):
         return RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, zero_tol=tol,
-                                        max_iter=1000, absorb_every=20, out=out)
+                                        max_iter=1000, absorb_every=ABSORB_MFMA, out=out)
 
     out = run_mfma()
     for _ in range(args.warmup):
@@ -219,8 +220,8 @@ def main():
             "workload": "batched log-stabilised Sinkhorn, one source vs many targets",
             "n": M, "targets": B, "support": int(nts.max()), "eps": reg,
             "stop": "L1 marginal error <= 1e-6 (FP64 on GPU)",
-            "precision": "BF16 MFMA products (FP32 accumulation), FP64 absorption every 20 "
-                         "iterations (FP64 MFMA marginals)",
+            "precision": "BF16 MFMA products (FP32 accumulation), FP64 absorption every "
+                         f"{ABSORB_MFMA} iterations (FP64 MFMA marginals)",
         },
         "converged": {"bf16_mfma": ok_mfma, "bf16_resident": okr, "fp32_resident": ok32,
                       "fp64_batched": ok64},

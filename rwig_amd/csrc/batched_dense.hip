@@ -21,7 +21,8 @@
 //     carry the O(1) part and the BF16 products only the corrections, whose
 //     rounding vanishes with (u - 1, v - 1) -- the fixed point is the FP64 one.
 // Absorption (FP64, every absorb_every iterations): f += reg log u,
-// g += reg log v (log u, log v are kept in FP32 between absorptions); r0 = alpha .* (K beta), c0 = beta .* (K^T alpha) by FP64
+// g += reg log v (between absorptions u, v are kept in FP32, as u - 1 and
+// v - 1 in the additive form); r0 = alpha .* (K beta), c0 = beta .* (K^T alpha) by FP64
 // MFMA GEMMs; the L1 marginal error of the absorbed plan decides convergence
 // (a target stops at the first absorption with error <= tol) and the form of
 // the next cycle.
@@ -170,6 +171,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   int *pos = alloc_into<int>(pool, (size_t)Nc * B);
   int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
   int *ndone = alloc_into<int>(pool, 1);
+  int *act = alloc_into<int>(pool, B);  // targets still iterating (GEMM column map)
   // GEMM operands
   uint16_t *W = alloc_into<uint16_t>(pool, (size_t)ldm * B);
   uint16_t *X = alloc_into<uint16_t>(pool, (size_t)ldn * B);
@@ -179,15 +181,16 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
 
   const uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
   const double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
+  int nact = B;
 
   // ---- initial state
   launch_map(st, MB, [=] __device__(long k) {
     F[k] = 0.0;
-    lu[k] = 0.0f;
+    lu[k] = 1.0f;
   });
   launch_map(st, (long)nnz, [=] __device__(long p) {
     G[p] = 0.0;
-    lv[p] = 0.0f;
+    lv[p] = 1.0f;
   });
   launch_map(st, (long)Nc * B, [=] __device__(long k) { pos[k] = -1; });
   launch_map(st, (long)nnz, [=] __device__(long p) {
@@ -212,7 +215,8 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
       if (!done[t]) {
-        const double fk = F[k] + reg * (double)lu[k];
+        const double x = (double)lu[k];
+        const double fk = F[k] + reg * (mode[t] ? log1p(x) : log(x));
         F[k] = fk;
         const double ak = exp(fk / reg);
         al[k] = (float)ak;
@@ -222,7 +226,8 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     launch_map(st, (long)nnz, [=] __device__(long p) {
       const long t = tgt_of[p];
       if (!done[t]) {
-        const double gq = G[p] + reg * (double)lv[p];
+        const double x = (double)lv[p];
+        const double gq = G[p] + reg * (mode[t] ? log1p(x) : log(x));
         G[p] = gq;
         const double bq = exp(gq / reg);
         be[p] = (float)bq;
@@ -230,14 +235,19 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       }
     });
     // exact marginals of the absorbed plan (FP64 MFMA)
-    gemm_tn<double>(st, M, B, Nc, KdT, ldn, Bd, ldn,
-                    [=] __device__(int i, int t, double v) {
-                      if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
-                    });
-    gemm_tn<double>(st, Nc, B, M, Kd, ldm, Ad, ldm, [=] __device__(int j, int t, double v) {
-      const int p = pos[j + (long)t * Nc];
-      if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
-    });
+    gemm_tn<double>(
+        st, M, nact, Nc, KdT, ldn, Bd, ldn,
+        [=] __device__(int i, int t, double v) {
+          if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
+        },
+        act);
+    gemm_tn<double>(
+        st, Nc, nact, M, Kd, ldm, Ad, ldm,
+        [=] __device__(int j, int t, double v) {
+          const int p = pos[j + (long)t * Nc];
+          if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
+        },
+        act);
     hipLaunchKernelGGL(k_dense_err, dim3(B), dim3(256), 0, st, M, B, a, r0, col_ptr, b, c0,
                        err_mode, tol, iter, maxiter, switch_thr, done, mode, iters, errs,
                        ndone);
@@ -250,30 +260,34 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     launch_map(st, (long)nnz, [=] __device__(long p) {
       X[col_idx[p] + (long)tgt_of[p] * ldn] = f2bf(be[p]);
     });
-    gemm_tn<uint16_t>(st, M, B, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
-      Yc[i + (long)t * M] = y;
-    });
-    gemm_tn<uint16_t>(st, Nc, B, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
-      const int p = pos[j + (long)t * Nc];
-      if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
-    });
+    gemm_tn<uint16_t>(
+        st, M, nact, Nc, KbT, ldn, X, ldn,
+        [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; }, act);
+    gemm_tn<uint16_t>(
+        st, Nc, nact, M, Kb, ldm, W, ldm,
+        [=] __device__(int j, int t, float z) {
+          const int p = pos[j + (long)t * Nc];
+          if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
+        },
+        act);
     // u = a / r0 with v = 1; W for the first product of the cycle
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
       if (done[t]) return;
       ra[k] = (float)(r0[k] / (double)Yc[k]);
       const double u = a[i] / r0[k];
-      lu[k] = (float)log(u);
+      lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
       W[i + t * ldm] = f2bf(mode[t] ? (float)((double)al[k] * (u - 1.0)) : (float)((double)al[k] * u));
     });
     launch_map(st, (long)nnz, [=] __device__(long p) {
-      if (!done[tgt_of[p]]) lv[p] = 0.0f;
+      const int t = tgt_of[p];
+      if (!done[t]) lv[p] = mode[t] ? 0.0f : 1.0f;
     });
   };
 
   auto iterate = [&]() {
     // Z = K^T W -> v, X
-    gemm_tn<uint16_t>(st, Nc, B, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
+    gemm_tn<uint16_t>(st, Nc, nact, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
       const int p = pos[j + (long)t * Nc];
       if (p < 0 || done[t]) return;
       double v;
@@ -284,10 +298,10 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
         v = b[p] / ((double)sb[p] * (double)z);
         X[j + (long)t * ldn] = f2bf((float)((double)be[p] * v));
       }
-      lv[p] = (float)log(v);
-    });
+      lv[p] = mode[t] ? (float)(v - 1.0) : (float)v;
+    }, act);
     // Y = K X -> u, W
-    gemm_tn<uint16_t>(st, M, B, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
+    gemm_tn<uint16_t>(st, M, nact, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
       if (done[t]) return;
       const long k = i + (long)t * M;
       double u;
@@ -298,16 +312,29 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
         u = a[i] / ((double)ra[k] * (double)y);
         W[i + (long)t * ldm] = f2bf((float)((double)al[k] * u));
       }
-      lu[k] = (float)log(u);
-    });
+      lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
+    }, act);
   };
 
+  // active list: every target at first; after each absorption the targets
+  // that stopped leave the GEMMs
+  std::vector<int> hact(B), hdone_v(B);
+  for (int t = 0; t < B; ++t) hact[t] = t;
+  h2d(act, hact.data(), B, st);
   int iter = 0, hdone = 0;
   absorb(iter);
   while (true) {
     d2h(&hdone, ndone, 1, st);
     RWIG_HIP_CHECK(hipStreamSynchronize(st));
     if (hdone >= B || iter >= maxiter) break;
+    if (hdone > B - nact) {
+      d2h(hdone_v.data(), done, B, st);
+      RWIG_HIP_CHECK(hipStreamSynchronize(st));
+      nact = 0;
+      for (int t = 0; t < B; ++t)
+        if (!hdone_v[t]) hact[nact++] = t;
+      h2d(act, hact.data(), nact, st);
+    }
     const int n = std::min(absorb_every, maxiter - iter);
     for (int s = 0; s < n; ++s) iterate();
     iter += n;

@@ -9,8 +9,8 @@
 // Workgroup tile BM x BN over k-steps of BK, staged through LDS in k-major
 // rows (each row of BK contiguous k values, padded against bank conflicts);
 // 4 wavefronts in a 2 x 2 grid, each computing a (BM/2) x (BN/2) block with
-// 16 x 16 MFMA tiles. The next k-step is loaded into registers while the
-// current one is multiplied. D is written through an epilogue functor
+// 16 x 16 MFMA tiles. The next k-step is loaded from global memory while the
+// current one is multiplied and stored into the second LDS stage after it. D is written through an epilogue functor
 // op(m, n, value) so callers can fuse their elementwise update.
 #pragma once
 
@@ -98,25 +98,36 @@ struct TileLoader {
   static constexpr int CHUNKS = ROWS * BK / VEC;      // 16-byte chunks per tile
   static constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
   uint4 reg[PER_THREAD];
+  const T *row_ptr[PER_THREAD];  // start of this thread's rows (null: outside)
 
-  __device__ void load(const T *__restrict__ p, long ld, int nrows, int K, int r0, int k0) {
+  // rows r0.. of a matrix with contiguous k (element (r, k) at p[r * ld + k]);
+  // rmap (optional): tile row r reads stored row rmap[r]. Resolved once.
+  __device__ void init(const T *__restrict__ p, long ld, int nrows, int r0,
+                       const int *__restrict__ rmap) {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (idx < CHUNKS) {
-        const int row = idx / (BK / VEC), kc = (idx % (BK / VEC)) * VEC;
-        const int r = r0 + row, k = k0 + kc;
-        if (r < nrows) {
-          const T *src = p + (long)r * ld + k;
-          if (k + VEC <= K) {
-            v = *reinterpret_cast<const uint4 *>(src);
-          } else {
-            T tmp[VEC];
+      const int r = r0 + idx / (BK / VEC);
+      row_ptr[c] = nullptr;
+      if (idx < CHUNKS && r < nrows) row_ptr[c] = p + (rmap ? (long)rmap[r] : (long)r) * ld;
+    }
+  }
+  // loads the tile's k values k0 .. k0 + BK (k >= K read as zero)
+  __device__ void load(int K, int k0) {
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) tmp[e] = (k + e < K) ? src[e] : T(0);
-            v = *reinterpret_cast<const uint4 *>(tmp);
-          }
+    for (int c = 0; c < PER_THREAD; ++c) {
+      const int idx = threadIdx.x + c * NT;
+      const int k = k0 + (idx % (BK / VEC)) * VEC;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (row_ptr[c]) {
+        const T *src = row_ptr[c] + k;
+        if (k + VEC <= K) {
+          v = *reinterpret_cast<const uint4 *>(src);
+        } else {
+          T tmp[VEC];
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) tmp[e] = (k + e < K) ? src[e] : T(0);
+          v = *reinterpret_cast<const uint4 *>(tmp);
         }
       }
       reg[c] = v;
@@ -137,14 +148,16 @@ struct TileLoader {
 template <typename T, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
-          long ldb, Op op) {
+          long ldb, const int *__restrict__ nmap, Op op) {
   using MF = Mfma<T>;
   constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN, BK = GemmCfg<T>::BK;
   constexpr int LDL = BK + GemmCfg<T>::PAD;
   constexpr int WM = BM / 2, WN = BN / 2;  // per-wave block
   constexpr int TM = WM / 16, TN = WN / 16;
-  __shared__ __attribute__((aligned(16))) T As[BM * LDL];
-  __shared__ __attribute__((aligned(16))) T Bs[BN * LDL];
+  // two LDS stages: the next k-step is stored into one while the other is
+  // being multiplied, one barrier per k-step
+  __shared__ __attribute__((aligned(16))) T As[2][BM * LDL];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDL];
 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wm = wv & 1, wn = wv >> 1;
@@ -158,30 +171,40 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 
   TileLoader<T, BM, BK, 256> la;
   TileLoader<T, BN, BK, 256> lb;
-  la.load(A, lda, M, K, m0, 0);
-  lb.load(B, ldb, N, K, n0, 0);
+  la.init(A, lda, M, m0, nullptr);
+  lb.init(B, ldb, N, n0, nmap);
+  la.load(K, 0);
+  lb.load(K, 0);
+  la.store(As[0], LDL);
+  lb.store(Bs[0], LDL);
+  __syncthreads();
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
+  int cur = 0;
   for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    la.store(As, LDL);
-    lb.store(Bs, LDL);
-    __syncthreads();
-    if (k0 + BK < K) {
-      la.load(A, lda, M, K, m0, k0 + BK);
-      lb.load(B, ldb, N, K, n0, k0 + BK);
+    const bool more = k0 + BK < K;
+    if (more) {
+      la.load(K, k0 + BK);
+      lb.load(K, k0 + BK);
     }
+    const T *as = As[cur], *bs = Bs[cur];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += MF::KI) {
       typename MF::frag_t af[TM], bfr[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = MF::load(As + (wm * WM + i * 16 + lr) * LDL + kk + lk);
+      for (int i = 0; i < TM; ++i) af[i] = MF::load(as + (wm * WM + i * 16 + lr) * LDL + kk + lk);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bfr[j] = MF::load(Bs + (wn * WN + j * 16 + lr) * LDL + kk + lk);
+      for (int j = 0; j < TN; ++j) bfr[j] = MF::load(bs + (wn * WN + j * 16 + lr) * LDL + kk + lk);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
     }
+    if (more) {
+      la.store(As[cur ^ 1], LDL);
+      lb.store(Bs[cur ^ 1], LDL);
+    }
+    __syncthreads();
+    cur ^= 1;
   }
   // accumulator lane layout: column lr, row MF::row(lane, r)
 #pragma unroll
@@ -189,23 +212,30 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const int n = n0 + wn * WN + j * 16 + lr;
+      if (n >= N) continue;
+      const int nn = nmap ? nmap[n] : n;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * WM + i * 16 + MF::row(lane, r);
-        if (m < M && n < N) op(m, n, acc[i][j][r]);
+        if (m < M) op(m, nn, acc[i][j][r]);
       }
     }
 }
 
+// nmap (optional, N entries): GEMM column n reads column nmap[n] of B and is
+// reported to the epilogue as column nmap[n] -- a product over a subset of
+// stored columns without moving them.
 template <typename T, class Op>
 void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
-             Op op) {
+             Op op, const int *nmap = nullptr) {
   constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
-  RWIG_REQUIRE(M > 0 && N > 0 && K > 0, "GEMM dimensions must be positive");
+  RWIG_REQUIRE(M > 0 && N >= 0 && K > 0, "GEMM dimensions must be positive");
   RWIG_REQUIRE(lda % (16 / sizeof(T)) == 0 && ldb % (16 / sizeof(T)) == 0,
                "GEMM leading dimensions must be multiples of 16 bytes");
+  if (N == 0) return;
   const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN);
-  hipLaunchKernelGGL((k_gemm_tn<T, Op>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, op);
+  hipLaunchKernelGGL((k_gemm_tn<T, Op>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, nmap,
+                     op);
   check_launch();
 }
 

@@ -83,6 +83,30 @@ int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A,
                         int S, double reg, int withgrad, int max_iter,
                         double zero_tol, int *return_status);
 
+/* rwig_barycenter_log with a choice of arithmetic and a checkpointed adjoint.
+ * Same iteration, outputs and stopping rule as rwig_barycenter_log
+ * (src/barycenter_impl_log.cpp:189-382); the forward keeps (F, G, row
+ * soft-min) every checkpoint_every iterations and the adjoint recomputes each
+ * segment from its checkpoint, so the adjoint's memory is
+ * max_iter/checkpoint_every + checkpoint_every + 2 slices of (M + N) x S
+ * instead of max_iter + 1.
+ *   precision 0: FP64 throughout (equal to rwig_barycenter_log);
+ *   precision 1: FP32 iterates and soft-min passes, FP64 gradient
+ *                accumulation, loss and adjoint seed.
+ *   seed_mode 0: the reference's adjoint seed 2 (b - b_ext) .* b, b
+ *                normalised (src/barycenter_impl_log.cpp:323-325);
+ *   seed_mode 1: exact gradient of ||b_L - b_ext||^2, b_L = exp(logb_L)
+ *                unnormalised (loss reported accordingly);
+ *   seed_mode 2: exact gradient of ||b - b_ext||^2 through the normalisation.
+ * Inputs and outputs are host FP64 arrays, column-major. */
+int rwig_barycenter_log_ck(double *F, double *G, double *b, double *grad_A,
+                           double *grad_w, double *loss, int *iter_out,
+                           double *err_out, const double *A, const double *w,
+                           const double *C, const double *b_ext, int M, int N,
+                           int S, double reg, int withgrad, int max_iter,
+                           double zero_tol, int checkpoint_every,
+                           int seed_mode, int precision, int *return_status);
+
 /* rwig_sinkhorn_log on DEVICE pointers of an already reduced problem (every
  * a_i and b_j > 0): the per-call plain FP64 form with inputs resident in HBM.
  * P (m x n), f (m), g (n), loss (1 scalar) and grad_a (m, withgrad) are
@@ -205,6 +229,56 @@ int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda,
 int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
                          const double *B, long ldb, double *D, long ldd,
                          void *stream);
+
+/* ---------------------------------------------------------------- serving
+ * A serving session: continuous batching of Sinkhorn and barycenter solves
+ * on cost matrices registered once and kept in HBM. No counterpart in the
+ * reference (one solve per call, every buffer allocated per call,
+ * src/cuda_sinkhorn.cu:217-328); the algorithms are the log-domain Sinkhorn
+ * (src/sinkhorn_impl_log.cpp:18-118) and the parallel barycenter
+ * (src/barycenter_impl_parallel.cpp:60-151), run as BF16 MFMA Gibbs products
+ * with FP64 absorption every absorb_every iterations (switch_thr as in
+ * rwig_sinkhorn_batched_mfma_dev).
+ *
+ * One engine thread per session owns every GPU launch. Requests on the same
+ * (cost, reg) share one GEMM per Gibbs product (one column per Sinkhorn
+ * problem, S per barycenter); different (cost, reg) pairs run on their own
+ * streams. Requests are admitted in arrival order while the session's HBM
+ * budget (cost matrices, cached Gibbs kernels, group columns) holds them.
+ * Stopping: L1 marginal error of the absorbed plans <= zero_tol, tested at
+ * each absorption, or max_iter reached (iteration counts are multiples of
+ * absorb_every). A request's result does not depend on the other requests
+ * in flight. Entries of a, b and A must be positive. Calls are thread-safe. */
+int rwig_serve_create(double hbm_budget_bytes, int max_group_columns,
+                      int absorb_every, double switch_thr, void **session);
+int rwig_serve_destroy(void *session);
+/* C: host, column-major M x N, copied to the device. */
+int rwig_serve_register_cost(void *session, const double *C, int M, int N,
+                             int *cost_id);
+int rwig_serve_submit_sinkhorn(void *session, int cost_id, double reg,
+                               const double *a, int M, const double *b, int N,
+                               int max_iter, double zero_tol, long *request_id);
+/* A: host, column-major M x S; w: S barycentric weights. */
+int rwig_serve_submit_barycenter(void *session, int cost_id, double reg,
+                                 const double *A, int M, int S, const double *w,
+                                 int max_iter, double zero_tol,
+                                 long *request_id);
+/* Waits until the request has finished (timeout_s < 0: no limit); state:
+ * 0 queued, 1 running, 2 done, 3 failed. */
+int rwig_serve_wait(void *session, long request_id, double timeout_s,
+                    int *state);
+/* Results of a finished request, which is then forgotten. Sinkhorn: x = f
+ * (M), y = g (N) potentials, loss = sum C.*P + reg sum P.*(log P - 1).
+ * Barycenter: x = b (N, normalised), y unused. latency_ms: submit to
+ * completion. */
+int rwig_serve_result(void *session, long request_id, double *x, double *y,
+                      double *loss, int *iter, double *err, double *latency_ms);
+/* out[0..14]: completed, failed, queued, running, p50 latency (ms),
+ * p99 latency (ms), mean latency (ms), solves/s (first submit to last
+ * completion), HBM bytes in use, peak HBM bytes, budget, engine cycles,
+ * groups, mean queueing delay (ms), max latency (ms). */
+int rwig_serve_stats(void *session, double *out, int n_out);
+int rwig_serve_reset_stats(void *session);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ SIGNATURES = {
                                   _i, _i, _i, _d, _i, _i, _d, _ip], _i),
     "rwig_barycenter_log": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _i,
                              _i, _i, _d, _i, _i, _d, _ip], _i),
+    "rwig_barycenter_log_ck": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
+                                _i, _i, _i, _d, _i, _i, _d, _i, _i, _i, _ip], _i),
     "rwig_sinkhorn_batched_dev": ([_vp, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp, _i, _d, _i,
                                    _d, _i, _i, _vp, ctypes.c_long, _i, _vp, _vp, _vp, _vp, _vp,
                                    _vp], _i),
@@ -53,6 +55,17 @@ SIGNATURES = {
     "rwig_wdl_batch_grad_dev": ([_vp, _vp, ctypes.c_long, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _i, _vp], _i),
     "rwig_gibbs_lowp_dev": ([_vp, ctypes.c_long, _i, _i, _d, _i, _vp, ctypes.c_long, _vp], _i),
+    "rwig_serve_create": ([_d, _i, _i, _d, ctypes.POINTER(ctypes.c_void_p)], _i),
+    "rwig_serve_destroy": ([_vp], _i),
+    "rwig_serve_register_cost": ([_vp, _dp, _i, _i, _ip], _i),
+    "rwig_serve_submit_sinkhorn": ([_vp, _i, _d, _dp, _i, _dp, _i, _i, _d,
+                                    ctypes.POINTER(ctypes.c_long)], _i),
+    "rwig_serve_submit_barycenter": ([_vp, _i, _d, _dp, _i, _i, _dp, _i, _d,
+                                      ctypes.POINTER(ctypes.c_long)], _i),
+    "rwig_serve_wait": ([_vp, ctypes.c_long, _d, _ip], _i),
+    "rwig_serve_result": ([_vp, ctypes.c_long, _dp, _dp, _dp, _ip, _dp, _dp], _i),
+    "rwig_serve_stats": ([_vp, _dp, _i], _i),
+    "rwig_serve_reset_stats": ([_vp], _i),
 }
 
 _lib = None

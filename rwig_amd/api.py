@@ -158,6 +158,56 @@ def barycenter(A, C, w, b_ext=None, barycenter_control=None):
     return out
 
 
+_SEEDS = {"reference": 0, "unnormalised": 1, "normalised": 2}
+
+
+def barycenter_log_ck(A, C, w, b_ext=None, reg=0.1, with_grad=False, max_iter=1000,
+                      zero_tol=1e-6, checkpoint_every=20, seed="reference", precision="fp64"):
+    """Log barycenter with a checkpointed adjoint, in FP64 or FP32.
+
+    Same iteration and outputs as barycenter(method="log")
+    (src/barycenter_impl_log.cpp:189-382). seed picks the adjoint's starting
+    gradient: "reference" (2 (b - b_ext) .* b, b normalised, as the
+    reference), "unnormalised" (exact gradient of ||exp(logb_L) - b_ext||^2)
+    or "normalised" (exact gradient of ||b - b_ext||^2 through the
+    normalisation).
+    """
+    A = _f64(A)
+    C = _f64(C)
+    w = _f64(w).ravel()
+    if A.ndim != 2 or C.ndim != 2 or A.shape[0] != C.shape[0]:
+        raise ValueError("A (M x S) and C (M x N) must be matrices with equal row counts")
+    if with_grad and b_ext is None:
+        raise ValueError("you must supply the `b_ext` with `with_grad = TRUE`!")
+    if seed not in _SEEDS:
+        raise ValueError("seed must be one of " + ", ".join(_SEEDS))
+    prec = {"fp64": 0, "fp32": 1}[precision]
+    M, S = A.shape
+    N = C.shape[1]
+    if w.size != S:
+        raise ValueError("length of w must equal the number of columns of A")
+    bext = _f64(b_ext).ravel() if with_grad else None
+    b = np.zeros(N)
+    F = np.zeros((M, S), order="F")
+    G = np.zeros((N, S), order="F")
+    gA = np.zeros((M, S), order="F")
+    gw = np.zeros(S)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    rc = ctypes.c_int()
+    check(lib().rwig_barycenter_log_ck(
+        _p(F), _p(G), _p(b), _p(gA), _p(gw), ctypes.byref(loss), ctypes.byref(it),
+        ctypes.byref(err), _p(A), _p(w), _p(C), _p(bext), M, N, S, float(reg), int(with_grad),
+        int(max_iter), float(zero_tol), int(checkpoint_every), _SEEDS[seed], prec,
+        ctypes.byref(rc)))
+    out = {"b": b, "F": F, "G": G, "iter": it.value, "err": err.value,
+           "return_status": rc.value}
+    if with_grad:
+        out.update(grad_A=gA, grad_w=gw, loss=loss.value)
+    return out
+
+
 def cuda_available():
     """cuda_available_cpp() (src/check_cuda.cpp:6-12): is a HIP device usable."""
     return bool(lib().rwig_available())

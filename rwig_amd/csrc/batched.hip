@@ -19,7 +19,7 @@
 // col_idx[col_ptr[t] .. col_ptr[t+1]) with masses b[] at the same positions.
 // Thread k of a workgroup owns rows k, k + TB, k + 2 TB, ... so every column
 // read is one contiguous, coalesced stretch of the column.
-#include "kernels_fp64.hpp"
+#include "kernels.hpp"
 
 namespace rwig {
 

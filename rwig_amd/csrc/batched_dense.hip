@@ -26,18 +26,13 @@
 // MFMA GEMMs; the L1 marginal error of the absorbed plan decides convergence
 // (a target stops at the first absorption with error <= tol) and the form of
 // the next cycle.
+#include "dense.hpp"
 #include "gemm.hpp"
-#include "kernels_fp64.hpp"
+#include "kernels.hpp"
 
 namespace rwig {
 
 namespace {
-
-__device__ __forceinline__ uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 
 // per-target L1 (or reference) marginal error of the absorbed plan, stopping
 // decision, form of the next cycle; one block per target
@@ -116,20 +111,12 @@ template <typename T> T *alloc_into(std::vector<DBuf<char>> &pool, size_t n) {
 
 }  // namespace
 
-// Shared-cost data of the dense path: K in BF16 and FP64, both orientations.
-struct DenseKernel {
-  int M = 0, Nc = 0;
-  long ldm = 0, ldn = 0;  // padded M, padded Nc (multiples of 8)
-  DBuf<uint16_t> Kb, KbT;  // Kb: Kb[i + j*ldm] = K(i,j); KbT: KbT[j + i*ldn] = K(i,j)
-  DBuf<double> Kd, KdT;
-};
-
 void dense_kernel_build(hipStream_t st, const double *C, long ldc, int M, int Nc, double reg,
                         DenseKernel &dk) {
   dk.M = M;
   dk.Nc = Nc;
-  dk.ldm = (M + 7) / 8 * 8;
-  dk.ldn = (Nc + 7) / 8 * 8;
+  dk.ldm = pad8(M);
+  dk.ldn = pad8(Nc);
   dk.Kb.reset((size_t)dk.ldm * Nc);
   dk.KbT.reset((size_t)dk.ldn * M);
   dk.Kd.reset((size_t)dk.ldm * Nc);

@@ -30,7 +30,7 @@
 // One workgroup owns one target; thread k owns rows 4(k + m TB) .. +3 for
 // m < RG so a wavefront's loads of a column are contiguous 8- or 16-byte
 // vectors.
-#include "kernels_fp64.hpp"
+#include "kernels.hpp"
 
 namespace rwig {
 

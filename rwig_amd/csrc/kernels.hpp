@@ -1,4 +1,5 @@
-// FP64 kernel family of the plain (reference-exact) GPU form.
+// Streaming kernel family of the plain GPU form: FP64 (reference-exact) and
+// FP32 (the soft-min passes, for the low-precision log-domain solves).
 //
 // Matrices are column-major. Every kernel that belongs to one iteration of an
 // iterative solve takes the solve's LoopState and returns at once when the
@@ -82,14 +83,14 @@ k_gemv_n_part(const double *__restrict__ K, int M, int N, long ldk,
 }
 
 // out index k = s*M + i: op(k, sum over chunks of part)
-template <class Op>
+template <typename T, class Op>
 __global__ void __launch_bounds__(kTB)
-k_reduce_part(const double *__restrict__ part, int nchunk, long len, Op op,
+k_reduce_part(const T *__restrict__ part, int nchunk, long len, Op op,
               const LoopState *ls) {
   if (ls && !ls->active) return;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
        k += (long)gridDim.x * blockDim.x) {
-    double s = 0.0;
+    T s = T(0);
     for (int c = 0; c < nchunk; ++c) s += part[(long)c * len + k];
     op(k, s);
   }
@@ -112,8 +113,8 @@ void gemv_n(hipStream_t st, const double *K, int M, int N, long ldk,
   }
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL(k_reduce_part<Op>, dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
-                     part, ch.nchunk, len, op, ls);
+  hipLaunchKernelGGL((k_reduce_part<double, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0,
+                     st, part, ch.nchunk, len, op, ls);
   check_launch();
 }
 
@@ -175,26 +176,25 @@ void gemv_t(hipStream_t st, const double *K, int M, int N, long ldk,
 // wavefront per column j, for every slice s:
 //   res[s] = min_i R - reg * log(sum_i exp(-(R - min_i R) / reg))
 // (the two-pass form of src/sinkhorn_impl_log.cpp:48-50).
-template <int SM, class Op>
+template <typename T, int SM, class Op>
 __global__ void __launch_bounds__(kTB)
-k_lse_col(const double *__restrict__ C, int M, int N, long ldc,
-          const double *__restrict__ F, long ldf, const double *__restrict__ G,
-          long ldg, int S, double reg, Op op, const LoopState *ls) {
+k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
+          const T *__restrict__ G, long ldg, int S, T reg, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (j >= N) return;
-  const double *col = C + (long)j * ldc;
-  double res[SM];
+  const T *col = C + (long)j * ldc;
+  T res[SM];
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
     if (s < S) {
-      const double *f = F + (long)s * ldf;
-      const double gj = G[j + (long)s * ldg];
-      double mn = INFINITY;
+      const T *f = F + (long)s * ldf;
+      const T gj = G[j + (long)s * ldg];
+      T mn = T(INFINITY);
       for (int i = lane; i < M; i += kWave) mn = fmin(mn, (col[i] - f[i]) - gj);
       mn = wave_min(mn);
-      double sm = 0.0;
+      T sm = T(0);
       for (int i = lane; i < M; i += kWave) sm += exp(-(((col[i] - f[i]) - gj) - mn) / reg);
       sm = wave_sum(sm);
       res[s] = mn - reg * log(sm);
@@ -203,50 +203,49 @@ k_lse_col(const double *__restrict__ C, int M, int N, long ldc,
   if (lane == 0) op(j, res);
 }
 
-template <class Op>
-void lse_col(hipStream_t st, const double *C, int M, int N, long ldc, const double *F,
-             long ldf, const double *G, long ldg, int S, double reg, Op op,
-             const LoopState *ls) {
+template <typename T, class Op>
+void lse_col(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
+             const T *G, long ldg, int S, double reg, Op op, const LoopState *ls) {
   const dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
-  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_lse_col<SM, Op>), grid, dim3(kTB), 0, st, C,
-                                            M, N, ldc, F, ldf, G, ldg, S, reg, op, ls));
+  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_lse_col<T, SM, Op>), grid, dim3(kTB), 0, st, C,
+                                            M, N, ldc, F, ldf, G, ldg, S, (T)reg, op, ls));
   check_launch();
 }
 
 // Row soft-min partials over the column chunk c: min and the shifted sum.
-static __global__ void __launch_bounds__(kTB)
-k_lse_row_part(const double *__restrict__ C, int M, int N, long ldc,
-               const double *__restrict__ F, long ldf, const double *__restrict__ G,
-               long ldg, int S, double reg, int cw, double *__restrict__ pm,
-               double *__restrict__ ps, const LoopState *ls) {
+template <typename T>
+__global__ void __launch_bounds__(kTB)
+k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
+               long ldf, const T *__restrict__ G, long ldg, int S, T reg, int cw,
+               T *__restrict__ pm, T *__restrict__ ps, const LoopState *ls) {
   if (ls && !ls->active) return;
   const int i = blockIdx.x * kTB + threadIdx.x;
   const int c = blockIdx.y, s = blockIdx.z;
   if (i >= M) return;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const double fi = F[i + (long)s * ldf];
-  const double *g = G + (long)s * ldg;
-  double mn = INFINITY;
+  const T fi = F[i + (long)s * ldf];
+  const T *g = G + (long)s * ldg;
+  T mn = T(INFINITY);
   for (int j = j0; j < j1; ++j) mn = fmin(mn, (C[i + j * ldc] - fi) - g[j]);
-  double sm = 0.0;
+  T sm = T(0);
   for (int j = j0; j < j1; ++j) sm += exp(-(((C[i + j * ldc] - fi) - g[j]) - mn) / reg);
   const long o = ((long)c * S + s) * M + i;
   pm[o] = mn;
   ps[o] = sm;
 }
 
-template <class Op>
+template <typename T, class Op>
 __global__ void __launch_bounds__(kTB)
-k_lse_row_merge(const double *__restrict__ pm, const double *__restrict__ ps, int nchunk,
-                long len, double reg, Op op, const LoopState *ls) {
+k_lse_row_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, long len, T reg,
+                Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
        k += (long)gridDim.x * blockDim.x) {
-    double m = pm[k];
+    T m = pm[k];
     for (int c = 1; c < nchunk; ++c) m = fmin(m, pm[(long)c * len + k]);
-    double t = 0.0;
+    T t = T(0);
     for (int c = 0; c < nchunk; ++c) {
-      const double mc = pm[(long)c * len + k];
+      const T mc = pm[(long)c * len + k];
       t += (mc == m) ? ps[(long)c * len + k] : ps[(long)c * len + k] * exp(-(mc - m) / reg);
     }
     op(k, m - reg * log(t));
@@ -255,18 +254,18 @@ k_lse_row_merge(const double *__restrict__ pm, const double *__restrict__ ps, in
 
 // Soft-min over columns j of R_s(i,j) for every row i and slice s:
 // op(k = s*M + i, value).
-template <class Op>
-void lse_row(hipStream_t st, const double *C, int M, int N, long ldc, const double *F,
-             long ldf, const double *G, long ldg, int S, double reg, double *pm, double *ps,
-             Op op, const LoopState *ls) {
+template <typename T, class Op>
+void lse_row(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
+             const T *G, long ldg, int S, double reg, T *pm, T *ps, Op op,
+             const LoopState *ls) {
   const Chunking ch = chunking(M, N);
   dim3 grid((M + kTB - 1) / kTB, ch.nchunk, S);
-  hipLaunchKernelGGL(k_lse_row_part, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S,
-                     reg, ch.cw, pm, ps, ls);
+  hipLaunchKernelGGL(k_lse_row_part<T>, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S,
+                     (T)reg, ch.cw, pm, ps, ls);
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL(k_lse_row_merge<Op>, dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
-                     pm, ps, ch.nchunk, len, reg, op, ls);
+  hipLaunchKernelGGL((k_lse_row_merge<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
+                     pm, ps, ch.nchunk, len, (T)reg, op, ls);
   check_launch();
 }
 
@@ -279,26 +278,25 @@ inline size_t lse_row_part_size(int M, int N, int S) {
 // Column pass with row soft-max weights (the adjoint X of the row soft-min):
 //   res[s] = sum_i exp(-(R_s(i,j) - rshift(i,s)) / reg) * x(i,s)
 // where rshift is the row soft-min of R_s, so the weights of each row sum to 1.
-template <int SM, class Op>
+template <typename T, int SM, class Op>
 __global__ void __launch_bounds__(kTB)
-k_wcol(const double *__restrict__ C, int M, int N, long ldc, const double *__restrict__ F,
-       long ldf, const double *__restrict__ G, long ldg, int S, double reg,
-       const double *__restrict__ rshift, const double *__restrict__ x, Op op,
-       const LoopState *ls) {
+k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
+       const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ rshift,
+       const T *__restrict__ x, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (j >= N) return;
-  const double *col = C + (long)j * ldc;
-  double res[SM];
+  const T *col = C + (long)j * ldc;
+  T res[SM];
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
     if (s < S) {
-      const double *f = F + (long)s * ldf;
-      const double *sh = rshift + (long)s * M;
-      const double *xs = x + (long)s * M;
-      const double gj = G[j + (long)s * ldg];
-      double acc = 0.0;
+      const T *f = F + (long)s * ldf;
+      const T *sh = rshift + (long)s * M;
+      const T *xs = x + (long)s * M;
+      const T gj = G[j + (long)s * ldg];
+      T acc = T(0);
       for (int i = lane; i < M; i += kWave)
         acc += exp(-(((col[i] - f[i]) - gj) - sh[i]) / reg) * xs[i];
       res[s] = wave_sum(acc);
@@ -307,51 +305,55 @@ k_wcol(const double *__restrict__ C, int M, int N, long ldc, const double *__res
   if (lane == 0) op(j, res);
 }
 
-template <class Op>
-void wcol(hipStream_t st, const double *C, int M, int N, long ldc, const double *F, long ldf,
-          const double *G, long ldg, int S, double reg, const double *rshift, const double *x,
-          Op op, const LoopState *ls) {
+template <typename T, class Op>
+void wcol(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf, const T *G,
+          long ldg, int S, double reg, const T *rshift, const T *x, Op op,
+          const LoopState *ls) {
   const dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
-  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_wcol<SM, Op>), grid, dim3(kTB), 0, st, C, M, N,
-                                            ldc, F, ldf, G, ldg, S, reg, rshift, x, op, ls));
+  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_wcol<T, SM, Op>), grid, dim3(kTB), 0, st, C, M,
+                                            N, ldc, F, ldf, G, ldg, S, (T)reg, rshift, x, op,
+                                            ls));
   check_launch();
 }
 
 // Row pass with column soft-max weights (the adjoint W of the column soft-min):
-//   part = sum_{j in chunk} exp(-(R_s(i,j) - cshift(j,s)) / reg) * y(j,s)
-static __global__ void __launch_bounds__(kTB)
-k_wrow_part(const double *__restrict__ C, int M, int N, long ldc,
-            const double *__restrict__ F, long ldf, const double *__restrict__ G, long ldg,
-            int S, double reg, const double *__restrict__ cshift,
-            const double *__restrict__ y, int cw, double *__restrict__ part,
+//   part = sum_{j in chunk} exp(-(R_s(i,j) - cshift(j,s) + rowsh(i,s)) / reg) * y(j,s)
+// rowsh (optional, M x S) scales row i by exp(-rowsh/reg) inside the
+// exponent, so rows whose result is tiny can be computed relative to their
+// own scale without underflow.
+template <typename T>
+__global__ void __launch_bounds__(kTB)
+k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
+            const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ cshift,
+            const T *__restrict__ y, const T *__restrict__ rowsh, int cw, T *__restrict__ part,
             const LoopState *ls) {
   if (ls && !ls->active) return;
   const int i = blockIdx.x * kTB + threadIdx.x;
   const int c = blockIdx.y, s = blockIdx.z;
   if (i >= M) return;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const double fi = F[i + (long)s * ldf];
-  const double *g = G + (long)s * ldg;
-  const double *sh = cshift + (long)s * N;
-  const double *ys = y + (long)s * N;
-  double acc = 0.0;
+  const T fi = rowsh ? F[i + (long)s * ldf] - rowsh[i + (long)s * M] : F[i + (long)s * ldf];
+  const T *g = G + (long)s * ldg;
+  const T *sh = cshift + (long)s * N;
+  const T *ys = y + (long)s * N;
+  T acc = T(0);
   for (int j = j0; j < j1; ++j)
     acc += exp(-(((C[i + j * ldc] - fi) - g[j]) - sh[j]) / reg) * ys[j];
   part[((long)c * S + s) * M + i] = acc;
 }
 
-template <class Op>
-void wrow(hipStream_t st, const double *C, int M, int N, long ldc, const double *F, long ldf,
-          const double *G, long ldg, int S, double reg, const double *cshift, const double *y,
-          double *part, Op op, const LoopState *ls) {
+template <typename T, class Op>
+void wrow(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf, const T *G,
+          long ldg, int S, double reg, const T *cshift, const T *y, T *part, Op op,
+          const LoopState *ls, const T *rowsh = nullptr) {
   const Chunking ch = chunking(M, N);
   dim3 grid((M + kTB - 1) / kTB, ch.nchunk, S);
-  hipLaunchKernelGGL(k_wrow_part, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S, reg,
-                     cshift, y, ch.cw, part, ls);
+  hipLaunchKernelGGL(k_wrow_part<T>, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S,
+                     (T)reg, cshift, y, rowsh, ch.cw, part, ls);
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL(k_reduce_part<Op>, dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st, part,
-                     ch.nchunk, len, op, ls);
+  hipLaunchKernelGGL((k_reduce_part<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
+                     part, ch.nchunk, len, op, ls);
   check_launch();
 }
 
@@ -403,6 +405,69 @@ void big_sum(hipStream_t st, long n, F f, double *part /* kSumBlocks */, double 
   reduce2(
       st, (long)kSumBlocks, [=] __device__(long k) { return part[k]; }, 0L,
       [=] __device__(long) { return 0.0; }, [=] __device__(double a, double) { *out = a; });
+}
+
+// Jacobi eigenvalue sweeps on the s x s symmetric matrix G (in place); returns
+// the largest eigenvalue. Same rotation sequence as the oracle's
+// spectral_norm().
+__device__ inline double jacobi_lmax(double *G, int s) {
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < s; ++p)
+      for (int q = p + 1; q < s; ++q) off += G[p + q * s] * G[p + q * s];
+    if (off == 0.0) break;
+    for (int p = 0; p < s; ++p)
+      for (int q = p + 1; q < s; ++q) {
+        const double apq = G[p + q * s];
+        if (apq == 0.0) continue;
+        const double app = G[p + p * s], aqq = G[q + q * s];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < s; ++k) {
+          const double gkp = G[k + p * s], gkq = G[k + q * s];
+          G[k + p * s] = c * gkp - sn * gkq;
+          G[k + q * s] = sn * gkp + c * gkq;
+        }
+        for (int k = 0; k < s; ++k) {
+          const double gpk = G[p + k * s], gqk = G[q + k * s];
+          G[p + k * s] = c * gpk - sn * gqk;
+          G[q + k * s] = sn * gpk + c * gqk;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int p = 0; p < s; ++p) lmax = fmax(lmax, G[p + p * s]);
+  return lmax;
+}
+
+// Spectral norm of the M x S matrix e(i, s) (Armadillo norm(X, 2),
+// src/barycenter_impl_log.cpp:252): Gram matrix by block sums, then Jacobi
+// on thread 0. fin(norm) runs on thread 0.
+template <class E, class Fin>
+__global__ void __launch_bounds__(1024)
+k_spectral(long M, int S, E e, Fin fin, const LoopState *gate) {
+  __shared__ double G[32 * 32];
+  __shared__ double sh[1024 / kWave];
+  if (gate && !gate->active) return;
+  for (int p = 0; p < S; ++p)
+    for (int q = p; q < S; ++q) {
+      double acc = 0.0;
+      for (long i = threadIdx.x; i < M; i += blockDim.x) acc += e(i, p) * e(i, q);
+      acc = block_sum(acc, sh);
+      if (threadIdx.x == 0) {
+        G[p + q * S] = acc;
+        G[q + p * S] = acc;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) fin(sqrt(jacobi_lmax(G, S)));
+}
+
+template <class E, class Fin>
+void spectral(hipStream_t st, long M, int S, E e, Fin fin, const LoopState *gate) {
+  hipLaunchKernelGGL((k_spectral<E, Fin>), dim3(1), dim3(1024), 0, st, M, S, e, fin, gate);
+  check_launch();
 }
 
 }  // namespace rwig

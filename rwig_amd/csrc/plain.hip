@@ -13,7 +13,7 @@
 // two passes over the matrix instead of the reference's four or five.
 #include <algorithm>
 
-#include "kernels_fp64.hpp"
+#include "kernels.hpp"
 
 namespace rwig {
 
@@ -56,69 +56,6 @@ __device__ __forceinline__ void finish_iteration(LoopState *ls, double err) {
 
 template <class T> void fill(hipStream_t st, T *p, long n, T v) {
   launch_map(st, n, [=] __device__(long k) { p[k] = v; });
-}
-
-// Jacobi eigenvalue sweeps on the s x s symmetric matrix G (in place); returns
-// the largest eigenvalue. Same rotation sequence as the oracle's
-// spectral_norm().
-__device__ double jacobi_lmax(double *G, int s) {
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < s; ++p)
-      for (int q = p + 1; q < s; ++q) off += G[p + q * s] * G[p + q * s];
-    if (off == 0.0) break;
-    for (int p = 0; p < s; ++p)
-      for (int q = p + 1; q < s; ++q) {
-        const double apq = G[p + q * s];
-        if (apq == 0.0) continue;
-        const double app = G[p + p * s], aqq = G[q + q * s];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < s; ++k) {
-          const double gkp = G[k + p * s], gkq = G[k + q * s];
-          G[k + p * s] = c * gkp - sn * gkq;
-          G[k + q * s] = sn * gkp + c * gkq;
-        }
-        for (int k = 0; k < s; ++k) {
-          const double gpk = G[p + k * s], gqk = G[q + k * s];
-          G[p + k * s] = c * gpk - sn * gqk;
-          G[q + k * s] = sn * gpk + c * gqk;
-        }
-      }
-  }
-  double lmax = 0.0;
-  for (int p = 0; p < s; ++p) lmax = fmax(lmax, G[p + p * s]);
-  return lmax;
-}
-
-// Spectral norm of the M x S matrix e(i, s) (Armadillo norm(X, 2),
-// src/barycenter_impl_log.cpp:252): Gram matrix by block sums, then Jacobi
-// on thread 0. fin(norm) runs on thread 0.
-template <class E, class Fin>
-__global__ void __launch_bounds__(1024)
-k_spectral(long M, int S, E e, Fin fin, const LoopState *gate) {
-  __shared__ double G[32 * 32];
-  __shared__ double sh[1024 / kWave];
-  if (gate && !gate->active) return;
-  for (int p = 0; p < S; ++p)
-    for (int q = p; q < S; ++q) {
-      double acc = 0.0;
-      for (long i = threadIdx.x; i < M; i += blockDim.x) acc += e(i, p) * e(i, q);
-      acc = block_sum(acc, sh);
-      if (threadIdx.x == 0) {
-        G[p + q * S] = acc;
-        G[q + p * S] = acc;
-      }
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) fin(sqrt(jacobi_lmax(G, S)));
-}
-
-template <class E, class Fin>
-void spectral(hipStream_t st, long M, int S, E e, Fin fin, const LoopState *gate) {
-  hipLaunchKernelGGL((k_spectral<E, Fin>), dim3(1), dim3(1024), 0, st, M, S, e, fin, gate);
-  check_launch();
 }
 
 // out(s) = scale * sum_j acc(j, s), one block per s.

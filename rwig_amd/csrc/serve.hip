@@ -1,0 +1,1070 @@
+// Serving engine: continuous batching of Sinkhorn and barycenter solves on
+// cost matrices kept resident in HBM.
+//
+// Requests arrive from any thread (submit) and are solved by one engine
+// thread that owns every GPU launch. Requests sharing a (cost, reg) pair run
+// in one group: each request owns columns of the group's GEMM operands (one
+// for a Sinkhorn problem, S for a barycenter of S measures) and every Gibbs
+// product of an iteration is one BF16 MFMA GEMM over the group's running
+// columns, with the same defect-corrected update and FP64 absorption as the
+// batched solver (multiplicative form far from convergence, additive form
+// near it; potentials, marginals and errors in FP64). Groups run on their own
+// streams, so the work of different cost matrices overlaps on the device.
+//
+// The engine works in cycles. At the start of a cycle, queued requests are
+// admitted while the HBM budget holds them: a new group costs its Gibbs
+// kernel (cached per (cost, reg), kept while the budget allows) and its column
+// buffers, a request its columns. Every group then absorbs (its requests'
+// stopping test), finished requests are harvested, and the remaining columns
+// run absorb_every iterations. A request's arithmetic depends only on its own
+// columns (the GEMMs read and write columns independently and every request
+// starts its absorption cycle at admission), so its result is the same
+// alone and in any mix.
+//
+// Algorithms: log-domain Sinkhorn (src/sinkhorn_impl_log.cpp:18-118) and the
+// parallel barycenter (src/barycenter_impl_parallel.cpp:60-151), written in
+// the absorbed form: potentials f, g in FP64 and scalings u, v relative to
+// them. For a barycenter the update of the common target is
+// b = prod_s (K^T U_s)^{w_s} with log(K^T U_s) = log q_s - g_s / reg, q_s the
+// column marginal of diag(u_s) P0_s, and v_s = b / q_s.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+#include "dense.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+namespace rwig {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kSinkhorn = 0, kBarycenter = 1;
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+constexpr int kColPage = 32;  // column capacity granularity of a group
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Stopping test of each running request at an absorption, one block per
+// request: L1 marginal error of the absorbed plans, sum over the request's
+// columns; for a barycenter the common target is re-estimated from the exact
+// column marginals, b = prod_s (c0_s e^{-g_s/reg})^{w_s}, and stored in the
+// request's first target column.
+__global__ void __launch_bounds__(256)
+k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
+            const int *__restrict__ riter, const int *__restrict__ kind,
+            const int *__restrict__ ns, const double *__restrict__ wcol,
+            const int *__restrict__ maxit, const double *__restrict__ tol,
+            const double *__restrict__ acol, const double *__restrict__ r0, double *bcol,
+            const double *__restrict__ c0, const double *__restrict__ G, double switch_thr,
+            int *done, int *mode, double *errs) {
+  __shared__ double sh[256 / kWave];
+  const int base = areq[blockIdx.x];
+  const int S = ns[base];
+  double e = 0.0;
+  for (int s = 0; s < S; ++s) {
+    const long o = (long)(base + s) * M;
+    for (int i = threadIdx.x; i < M; i += blockDim.x) e += fabs(r0[o + i] - acol[o + i]);
+  }
+  if (kind[base] == kSinkhorn) {
+    const long o = (long)base * N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) e += fabs(c0[o + j] - bcol[o + j]);
+  } else {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      double lb = 0.0;
+      for (int s = 0; s < S; ++s) {
+        const long q = j + (long)(base + s) * N;
+        lb += wcol[base + s] * (log(c0[q]) - G[q] / reg);
+      }
+      const double bj = exp(lb);
+      for (int s = 0; s < S; ++s) e += fabs(c0[j + (long)(base + s) * N] - bj);
+      bcol[j + (long)base * N] = bj;
+    }
+  }
+  e = block_sum(e, sh);
+  if (threadIdx.x == 0) {
+    const int it = riter[blockIdx.x];
+    errs[base] = e;
+    const int stop = (it < maxit[base] && e >= tol[base]) ? 0 : 1;
+    const int md = (e < switch_thr) ? 1 : 0;
+    for (int s = 0; s < S; ++s) {
+      done[base + s] = stop;
+      mode[base + s] = md;
+    }
+  }
+}
+
+struct Cost {
+  int M = 0, N = 0;
+  DBuf<double> C;
+};
+
+struct Gibbs {
+  int cost = -1;
+  double reg = 0.0;
+  DenseKernel dk;
+  size_t bytes = 0;
+  int users = 0;      // groups using it
+  uint64_t last = 0;  // engine cycle of the last use
+};
+
+struct Group;
+
+struct Request {
+  long id = 0;
+  int kind = kSinkhorn, cost = 0, S = 1, max_iter = 0;
+  double reg = 0.0, tol = 0.0;
+  std::vector<double> in1, in2;  // Sinkhorn: a (M), b (N); barycenter: A (M x S), w (S)
+  std::vector<double> out1, out2;  // Sinkhorn: f (M), g (N); barycenter: b (N)
+  double loss = 0.0, err = 0.0;
+  int iter = 0;
+  int state = kQueued;
+  std::string error;
+  Clock::time_point t_submit, t_admit, t_done;
+  Group *group = nullptr;
+  int col = -1;
+};
+
+// Column-major device array with a fixed number of bytes per column.
+struct Slab {
+  DBuf<char> buf;
+  size_t per_col = 0;
+  template <class T> T *p() const { return reinterpret_cast<T *>(buf.get()); }
+};
+
+// Requests of one (cost, reg) pair and their columns.
+struct Group {
+  int cost = -1;
+  double reg = 0.0;
+  Gibbs *gk = nullptr;
+  const double *C = nullptr;
+  int M = 0, N = 0;
+  long ldm = 0, ldn = 0;
+  int cap = 0;
+  std::unique_ptr<Stream> stream;  // created with the group
+  hipEvent_t ev = nullptr;
+  uint64_t last = 0;
+  std::vector<Request *> owner;  // column -> request occupying it
+  std::vector<Request *> running;
+  // per-column device state
+  Slab acol, F, r0, lu, al, ra, Yc, W, Ad;   // source side (M rows)
+  Slab bcol, G, c0, lv, be, sb, Zq, X, Bd;   // target side (N rows)
+  Slab kind, base, ns, done, mode, maxit;    // ints per column
+  Slab wcol, tol, errs;                      // doubles per column
+  // column and request lists: act | areq | riter | abar, 4 x cap ints
+  Slab lists;
+  int *hl[2] = {nullptr, nullptr};  // pinned host copies, one per upload phase
+  DBuf<double> part, scal;          // loss reduction partials and result
+  int *hdone_pin = nullptr;
+  double *herr_pin = nullptr;
+  int nact = 0, nreq = 0, nbar = 0;
+
+  std::vector<Slab *> slabs() {
+    return {&acol, &F, &r0, &lu, &al, &ra, &Yc, &W, &Ad, &bcol, &G, &c0, &lv, &be, &sb,
+            &Zq, &X, &Bd, &kind, &base, &ns, &done, &mode, &maxit, &wcol, &tol, &errs,
+            &lists};
+  }
+
+  void layout() {
+    acol.per_col = F.per_col = r0.per_col = 8 * (size_t)M;
+    lu.per_col = al.per_col = ra.per_col = Yc.per_col = 4 * (size_t)M;
+    W.per_col = 2 * (size_t)ldm;
+    Ad.per_col = 8 * (size_t)ldm;
+    bcol.per_col = G.per_col = c0.per_col = 8 * (size_t)N;
+    lv.per_col = be.per_col = sb.per_col = Zq.per_col = 4 * (size_t)N;
+    X.per_col = 2 * (size_t)ldn;
+    Bd.per_col = 8 * (size_t)ldn;
+    kind.per_col = base.per_col = ns.per_col = done.per_col = mode.per_col = maxit.per_col = 4;
+    wcol.per_col = tol.per_col = errs.per_col = 8;
+    lists.per_col = 16;
+  }
+
+  size_t per_col_bytes() {
+    size_t b = 0;
+    for (Slab *s : slabs()) b += s->per_col;
+    return b + 2 * 16 + 12;  // pinned lists, flags and errors
+  }
+
+  ~Group() {
+    if (ev) (void)hipEventDestroy(ev);
+    for (int *&h : hl)
+      if (h) (void)hipHostFree(h);
+    if (hdone_pin) (void)hipHostFree(hdone_pin);
+    if (herr_pin) (void)hipHostFree(herr_pin);
+  }
+
+  // Grows every per-column array to ncap columns, keeping the first cap
+  // columns (column-major arrays: a prefix copy). The stream is drained
+  // first: the old buffers go back to the allocator cache.
+  void grow(int ncap) {
+    stream->sync();
+    for (Slab *s : slabs()) {
+      DBuf<char> nb(s->per_col * (size_t)ncap);
+      if (cap) d2d(nb.get(), s->buf.get(), s->per_col * (size_t)cap, stream->get());
+      stream->sync();
+      s->buf = std::move(nb);
+    }
+    for (int *&h : hl) {
+      if (h) RWIG_HIP_CHECK(hipHostFree(h));
+      RWIG_HIP_CHECK(hipHostMalloc((void **)&h, sizeof(int) * 4 * (size_t)ncap));
+    }
+    if (hdone_pin) RWIG_HIP_CHECK(hipHostFree(hdone_pin));
+    if (herr_pin) RWIG_HIP_CHECK(hipHostFree(herr_pin));
+    RWIG_HIP_CHECK(hipHostMalloc((void **)&hdone_pin, sizeof(int) * (size_t)ncap));
+    RWIG_HIP_CHECK(hipHostMalloc((void **)&herr_pin, sizeof(double) * (size_t)ncap));
+    owner.resize(ncap, nullptr);
+    cap = ncap;
+  }
+
+  // First run of S free columns, or -1.
+  int find_cols(int S) const {
+    for (int c = 0, run = 0; c < cap; ++c) {
+      run = owner[c] ? 0 : run + 1;
+      if (run == S) return c - S + 1;
+    }
+    return -1;
+  }
+
+  // Uploads act | areq | riter | abar from the host lists (phase 0: before an
+  // absorption, 1: before iterations). Each phase has its own pinned buffer,
+  // which is rewritten only after a stream synchronisation that follows its
+  // previous upload.
+  void upload_lists(int phase, bool only_running) {
+    int *h = hl[phase];
+    int *hact = h, *hreq = h + cap, *hit = h + 2 * cap, *hbar = h + 3 * cap;
+    nact = nreq = nbar = 0;
+    for (Request *r : running) {
+      if (only_running && r->state != kRunning) continue;
+      hreq[nreq] = r->col;
+      hit[nreq] = r->iter;
+      ++nreq;
+      if (r->kind == kBarycenter) hbar[nbar++] = r->col;
+      for (int s = 0; s < r->S; ++s) hact[nact++] = r->col + s;
+    }
+    h2d(lists.p<int>(), h, 4 * (size_t)cap, stream->get());
+  }
+  const int *d_act() const { return lists.p<int>(); }
+  const int *d_areq() const { return lists.p<int>() + cap; }
+  const int *d_riter() const { return lists.p<int>() + 2 * cap; }
+  const int *d_abar() const { return lists.p<int>() + 3 * cap; }
+
+  // Fresh state of the columns [c0, c0 + S) of a newly admitted request.
+  void init_request(const Request &r) {
+    hipStream_t st = stream->get();
+    const int c0i = r.col, S = r.S, Mi = M, Ni = N;
+    const long ldmi = ldm, ldni = ldn;
+    if (r.kind == kSinkhorn) {
+      h2d(acol.p<double>() + (long)c0i * M, r.in1.data(), (size_t)M, st);
+      h2d(bcol.p<double>() + (long)c0i * N, r.in2.data(), (size_t)N, st);
+      const double one = 1.0;
+      h2d(wcol.p<double>() + c0i, &one, 1, st);
+    } else {
+      h2d(acol.p<double>() + (long)c0i * M, r.in1.data(), (size_t)M * S, st);
+      h2d(wcol.p<double>() + c0i, r.in2.data(), (size_t)S, st);
+    }
+    double *Fp = F.p<double>(), *Adp = Ad.p<double>(), *Gp = G.p<double>(), *Bdp = Bd.p<double>();
+    float *lup = lu.p<float>(), *lvp = lv.p<float>();
+    uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
+    launch_map(st, ldmi * S, [=] __device__(long k) {
+      const long i = k % ldmi, t = c0i + k / ldmi;
+      Wp[i + t * ldmi] = 0;
+      Adp[i + t * ldmi] = 0.0;
+      if (i < Mi) {
+        Fp[i + t * Mi] = 0.0;
+        lup[i + t * Mi] = 1.0f;
+      }
+    });
+    launch_map(st, ldni * S, [=] __device__(long k) {
+      const long j = k % ldni, t = c0i + k / ldni;
+      Xp[j + t * ldni] = 0;
+      Bdp[j + t * ldni] = 0.0;
+      if (j < Ni) {
+        Gp[j + t * Ni] = 0.0;
+        lvp[j + t * Ni] = 1.0f;
+      }
+    });
+    int *kp = kind.p<int>(), *bp = base.p<int>(), *np = ns.p<int>(), *dp = done.p<int>(),
+        *mp = mode.p<int>(), *xp = maxit.p<int>();
+    double *tp = tol.p<double>();
+    const int kd = r.kind, mx = r.max_iter;
+    const double tl = r.tol;
+    launch_map(st, (long)S, [=] __device__(long s) {
+      const long c = c0i + s;
+      kp[c] = kd;
+      bp[c] = c0i;
+      np[c] = S;
+      dp[c] = 0;
+      mp[c] = 0;
+      xp[c] = mx;
+      tp[c] = tl;
+    });
+  }
+
+  // Fold the scalings into the potentials, exact FP64 marginals, stopping
+  // test, calibration of the BF16 products and reset of the scalings.
+  void absorb(double switch_thr) {
+    hipStream_t st = stream->get();
+    const int Mi = M, Ni = N, na = nact;
+    const long ldmi = ldm, ldni = ldn;
+    const double rg = reg;
+    const int *act = d_act();
+    double *Fp = F.p<double>(), *Gp = G.p<double>(), *Adp = Ad.p<double>(),
+           *Bdp = Bd.p<double>(), *r0p = r0.p<double>(), *c0p = c0.p<double>(),
+           *ap = acol.p<double>(), *bp = bcol.p<double>();
+    float *lup = lu.p<float>(), *lvp = lv.p<float>(), *alp = al.p<float>(), *bep = be.p<float>(),
+          *rap = ra.p<float>(), *sbp = sb.p<float>(), *Ycp = Yc.p<float>();
+    uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
+    int *dn = done.p<int>(), *md = mode.p<int>();
+    const DenseKernel &dk = gk->dk;
+    launch_map(st, (long)Mi * na, [=] __device__(long k) {
+      const long t = act[k / Mi], i = k % Mi, q = i + t * Mi;
+      const double x = (double)lup[q];
+      const double fk = Fp[q] + rg * (md[t] ? log1p(x) : log(x));
+      Fp[q] = fk;
+      const double ak = exp(fk / rg);
+      alp[q] = (float)ak;
+      Adp[i + t * ldmi] = ak;
+    });
+    launch_map(st, (long)Ni * na, [=] __device__(long k) {
+      const long t = act[k / Ni], j = k % Ni, q = j + t * Ni;
+      const double x = (double)lvp[q];
+      const double gq = Gp[q] + rg * (md[t] ? log1p(x) : log(x));
+      Gp[q] = gq;
+      const double bq = exp(gq / rg);
+      bep[q] = (float)bq;
+      Bdp[j + t * ldni] = bq;
+    });
+    gemm_tn<double>(
+        st, Mi, na, Ni, dk.KdT.get(), ldni, Bdp, ldni,
+        [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
+        act);
+    gemm_tn<double>(
+        st, Ni, na, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
+        [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
+        act);
+    hipLaunchKernelGGL(k_serve_err, dim3(nreq), dim3(256), 0, st, Mi, Ni, rg, d_areq(),
+                       d_riter(), kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
+                       tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,
+                       errs.p<double>());
+    check_launch();
+    // calibration: the same BF16 products at u = v = 1
+    launch_map(st, (long)Mi * na, [=] __device__(long k) {
+      const long t = act[k / Mi], i = k % Mi;
+      Wp[i + t * ldmi] = f2bf(alp[i + t * Mi]);
+    });
+    launch_map(st, (long)Ni * na, [=] __device__(long k) {
+      const long t = act[k / Ni], j = k % Ni;
+      Xp[j + t * ldni] = f2bf(bep[j + t * Ni]);
+    });
+    gemm_tn<uint16_t>(
+        st, Mi, na, Ni, dk.KbT.get(), ldni, Xp, ldni,
+        [=] __device__(int i, int t, float y) {
+          if (!dn[t]) Ycp[i + (long)t * Mi] = y;
+        },
+        act);
+    gemm_tn<uint16_t>(
+        st, Ni, na, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+        [=] __device__(int j, int t, float z) {
+          const long q = j + (long)t * Ni;
+          if (!dn[t]) sbp[q] = (float)(c0p[q] / (double)z);
+        },
+        act);
+    // u = a / r0 at v = 1; W for the first product of the cycle
+    launch_map(st, (long)Mi * na, [=] __device__(long k) {
+      const long t = act[k / Mi], i = k % Mi, q = i + t * Mi;
+      if (dn[t]) return;
+      rap[q] = (float)(r0p[q] / (double)Ycp[q]);
+      const double u = ap[q] / r0p[q];
+      const double a_ = (double)alp[q];
+      lup[q] = md[t] ? (float)(u - 1.0) : (float)u;
+      Wp[i + t * ldmi] = f2bf(md[t] ? (float)(a_ * (u - 1.0)) : (float)(a_ * u));
+    });
+    launch_map(st, (long)Ni * na, [=] __device__(long k) {
+      const long t = act[k / Ni], j = k % Ni;
+      if (!dn[t]) lvp[j + t * Ni] = md[t] ? 0.0f : 1.0f;
+    });
+    d2h(hdone_pin, dn, (size_t)cap, st);
+    d2h(herr_pin, errs.p<double>(), (size_t)cap, st);
+    RWIG_HIP_CHECK(hipEventRecord(ev, st));
+  }
+
+  // One iteration over the running columns: Z = K^T W -> v, X; for the
+  // barycenters the common target and v; Y = K X -> u, W.
+  void iterate() {
+    hipStream_t st = stream->get();
+    const int Mi = M, Ni = N, na = nact, nb = nbar;
+    const long ldmi = ldm, ldni = ldn;
+    const double rg = reg;
+    const int *act = d_act(), *abar = d_abar();
+    const int *kd = kind.p<int>(), *md = mode.p<int>(), *np = ns.p<int>();
+    const double *c0p = c0.p<double>(), *r0p = r0.p<double>(), *ap = acol.p<double>(),
+                 *bp = bcol.p<double>(), *Gp = G.p<double>(), *wp = wcol.p<double>();
+    const float *bep = be.p<float>(), *sbp = sb.p<float>(), *alp = al.p<float>(),
+                *rap = ra.p<float>();
+    float *lvp = lv.p<float>(), *lup = lu.p<float>(), *Zqp = Zq.p<float>();
+    uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
+    const DenseKernel &dk = gk->dk;
+    gemm_tn<uint16_t>(
+        st, Ni, na, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+        [=] __device__(int j, int t, float z) {
+          const long q = j + (long)t * Ni;
+          if (kd[t] == kBarycenter) {
+            Zqp[q] = z;
+            return;
+          }
+          const double b_ = (double)bep[q];
+          double v;
+          if (md[t]) {
+            v = bp[q] / (c0p[q] + b_ * (double)z);
+            Xp[j + (long)t * ldni] = f2bf((float)(b_ * (v - 1.0)));
+            lvp[q] = (float)(v - 1.0);
+          } else {
+            v = bp[q] / ((double)sbp[q] * (double)z);
+            Xp[j + (long)t * ldni] = f2bf((float)(b_ * v));
+            lvp[q] = (float)v;
+          }
+        },
+        act);
+    if (nb > 0)
+      launch_map(st, (long)Ni * nb, [=] __device__(long k) {
+        const long r = abar[k / Ni], j = k % Ni;
+        const int S = np[r], m1 = md[r];
+        double lb = 0.0;
+        for (int s = 0; s < S; ++s) {
+          const long q = j + (r + s) * Ni;
+          const double z = (double)Zqp[q];
+          const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
+          lb += wp[r + s] * (log(qs) - Gp[q] / rg);
+        }
+        const double bj = exp(lb);
+        for (int s = 0; s < S; ++s) {
+          const long q = j + (r + s) * Ni;
+          const double z = (double)Zqp[q];
+          const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
+          const double v = bj / qs;
+          const double b_ = (double)bep[q];
+          Xp[j + (r + s) * ldni] = f2bf(m1 ? (float)(b_ * (v - 1.0)) : (float)(b_ * v));
+          lvp[q] = m1 ? (float)(v - 1.0) : (float)v;
+        }
+      });
+    gemm_tn<uint16_t>(
+        st, Mi, na, Ni, dk.KbT.get(), ldni, Xp, ldni,
+        [=] __device__(int i, int t, float y) {
+          const long q = i + (long)t * Mi;
+          const double a_ = (double)alp[q];
+          double u;
+          if (md[t]) {
+            u = ap[q] / (r0p[q] + a_ * (double)y);
+            Wp[i + (long)t * ldmi] = f2bf((float)(a_ * (u - 1.0)));
+            lup[q] = (float)(u - 1.0);
+          } else {
+            u = ap[q] / ((double)rap[q] * (double)y);
+            Wp[i + (long)t * ldmi] = f2bf((float)(a_ * u));
+            lup[q] = (float)u;
+          }
+        },
+        act);
+  }
+
+  // Results of a finished request into its host vectors (stream synchronised).
+  void harvest(Request &r) {
+    hipStream_t st = stream->get();
+    const int Mi = M, Ni = N, c = r.col;
+    const double rg = reg;
+    r.err = herr_pin[c];
+    if (r.kind == kSinkhorn) {
+      // regularised loss sum C.*P + reg sum P.*(log P - 1) of the absorbed
+      // plan (src/sinkhorn_impl_log.cpp:95-101)
+      const double *Cp = C, *Fp = F.p<double>() + (long)c * Mi, *Gp = G.p<double>() + (long)c * Ni;
+      big_sum(
+          st, (long)Mi * Ni,
+          [=] __device__(long k) {
+            const long i = k % Mi, j = k / Mi;
+            const double cij = Cp[k];
+            const double lp = (Fp[i] + Gp[j] - cij) / rg;
+            const double p = exp(lp);
+            return cij * p + rg * p * (lp - 1.0);
+          },
+          part.get(), scal.get());
+      r.out1.resize(Mi);
+      r.out2.resize(Ni);
+      d2h(r.out1.data(), Fp, (size_t)Mi, st);
+      d2h(r.out2.data(), Gp, (size_t)Ni, st);
+      d2h(&r.loss, scal.get(), 1, st);
+    } else {
+      r.out1.resize(Ni);
+      d2h(r.out1.data(), bcol.p<double>() + (long)c * Ni, (size_t)Ni, st);
+    }
+    stream->sync();
+    if (r.kind == kBarycenter) {
+      double t = 0.0;
+      for (double x : r.out1) t += x;
+      for (double &x : r.out1) x /= t;  // src/barycenter_impl_parallel.cpp:39-43
+    }
+  }
+};
+
+}  // namespace
+
+class ServeEngine {
+ public:
+  ServeEngine(double budget, int max_cols, int absorb_every, double switch_thr)
+      : budget_(budget), max_cols_(max_cols), absorb_every_(absorb_every),
+        switch_thr_(switch_thr) {
+    RWIG_HIP_CHECK(hipGetDevice(&device_));
+    th_ = std::thread([this] { loop(); });
+  }
+
+  ~ServeEngine() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    th_.join();
+  }
+
+  int register_cost(const double *C, int M, int N) {
+    const size_t bytes = (size_t)M * N * sizeof(double);
+    auto c = std::make_unique<Cost>();
+    c->M = M;
+    c->N = N;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      RWIG_REQUIRE(fatal_.empty(), "serving session failed: " + fatal_);
+      RWIG_REQUIRE(used_ + bytes <= budget_, "cost matrix exceeds the session's HBM budget");
+      used_ += bytes;
+      cost_bytes_ += bytes;
+      peak_ = std::max(peak_, used_);
+    }
+    try {
+      Stream s;
+      c->C.reset((size_t)M * N);
+      h2d(c->C.get(), C, (size_t)M * N, s.get());
+      s.sync();
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mu_);
+      used_ -= bytes;
+      cost_bytes_ -= bytes;
+      throw;
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    costs_.push_back(std::move(c));
+    return (int)costs_.size() - 1;
+  }
+
+  long submit(std::unique_ptr<Request> r) {
+    std::lock_guard<std::mutex> lk(mu_);
+    RWIG_REQUIRE(fatal_.empty(), "serving session failed: " + fatal_);
+    RWIG_REQUIRE(r->cost >= 0 && r->cost < (int)costs_.size(), "unknown cost id");
+    const Cost &c = *costs_[r->cost];
+    if (r->kind == kSinkhorn)
+      RWIG_REQUIRE((int)r->in1.size() == c.M && (int)r->in2.size() == c.N,
+                   "a and b must match the cost matrix");
+    else
+      RWIG_REQUIRE((int)r->in1.size() == c.M * r->S, "A must have M rows");
+    RWIG_REQUIRE(r->S <= max_cols_, "request has more measures than a group holds");
+    const size_t need = alone_bytes(c.M, c.N, r->S);
+    RWIG_REQUIRE(cost_bytes_ + need <= budget_,
+                 "request does not fit the session's HBM budget even alone");
+    r->id = next_id_++;
+    r->t_submit = Clock::now();
+    if (!started_) {
+      t_first_ = r->t_submit;
+      started_ = true;
+    }
+    const long id = r->id;
+    Request *rp = r.get();
+    reqs_[id] = std::move(r);
+    incoming_.push_back(rp);
+    ++queued_;
+    cv_work_.notify_all();
+    return id;
+  }
+
+  int wait(long id, double timeout_s) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = reqs_.find(id);
+    RWIG_REQUIRE(it != reqs_.end(), "unknown request id");
+    Request *r = it->second.get();
+    auto pred = [&] { return r->state == kDone || r->state == kFailed; };
+    if (timeout_s < 0)
+      cv_done_.wait(lk, pred);
+    else
+      cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+    return r->state;
+  }
+
+  std::unique_ptr<Request> take(long id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = reqs_.find(id);
+    RWIG_REQUIRE(it != reqs_.end(), "unknown request id");
+    Request *r = it->second.get();
+    RWIG_REQUIRE(r->state == kDone || r->state == kFailed, "request has not finished");
+    std::unique_ptr<Request> out = std::move(it->second);
+    reqs_.erase(it);
+    return out;
+  }
+
+  void stats(double *out, int n) {
+    std::lock_guard<std::mutex> lk(mu_);
+    std::vector<double> lat = lat_;
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) {
+      if (lat.empty()) return 0.0;
+      const double x = p * (lat.size() - 1);
+      const size_t lo = (size_t)x, hi = std::min(lo + 1, lat.size() - 1);
+      return lat[lo] + (x - lo) * (lat[hi] - lat[lo]);
+    };
+    double mean = 0.0, qmean = 0.0;
+    for (double x : lat) mean += x;
+    for (double x : qwait_) qmean += x;
+    if (!lat.empty()) mean /= lat.size();
+    if (!qwait_.empty()) qmean /= qwait_.size();
+    const double span = (completed_ > 0) ? ms_between(t_first_, t_last_) / 1e3 : 0.0;
+    const double v[] = {(double)completed_,
+                        (double)failed_,
+                        (double)queued_,
+                        (double)running_,
+                        pct(0.50),
+                        pct(0.99),
+                        mean,
+                        span > 0 ? completed_ / span : 0.0,
+                        (double)used_,
+                        (double)peak_,
+                        budget_,
+                        (double)cycles_,
+                        (double)ngroups_,
+                        qmean,
+                        lat.empty() ? 0.0 : lat.back()};
+    const int nv = (int)(sizeof(v) / sizeof(v[0]));
+    for (int k = 0; k < n; ++k) out[k] = k < nv ? v[k] : 0.0;
+  }
+
+  void reset_stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    lat_.clear();
+    qwait_.clear();
+    completed_ = failed_ = 0;
+    started_ = false;
+    peak_ = used_;
+    cycles_ = 0;
+  }
+
+ private:
+  // HBM bytes of a request running alone: its Gibbs kernel and one page of
+  // group columns (at least S).
+  size_t alone_bytes(int M, int N, int S) {
+    Group probe;
+    probe.M = M;
+    probe.N = N;
+    probe.ldm = pad8(M);
+    probe.ldn = pad8(N);
+    probe.layout();
+    const int cols = std::max(kColPage, S);
+    return dense_kernel_bytes(M, N) + probe.per_col_bytes() * cols;
+  }
+
+  // Makes room for `need` bytes by releasing idle groups, then idle Gibbs
+  // kernels, least recently used first (engine thread).
+  bool make_room(size_t need, const Gibbs *keep) {
+    auto used = [&] {
+      std::lock_guard<std::mutex> lk(mu_);
+      return used_;
+    };
+    while (used() + need > budget_) {
+      Group *gv = nullptr;
+      for (auto &g : groups_)
+        if (g->running.empty() && (!gv || g->last < gv->last)) gv = g.get();
+      if (gv) {
+        release_group(gv);
+        continue;
+      }
+      Gibbs *kv = nullptr;
+      for (auto &k : gibbs_)
+        if (k->users == 0 && k.get() != keep && (!kv || k->last < kv->last)) kv = k.get();
+      if (!kv) return false;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        used_ -= kv->bytes;
+      }
+      gibbs_.erase(std::find_if(gibbs_.begin(), gibbs_.end(),
+                                [&](const std::unique_ptr<Gibbs> &p) { return p.get() == kv; }));
+    }
+    return true;
+  }
+
+  void charge(long delta) {
+    std::lock_guard<std::mutex> lk(mu_);
+    used_ = (size_t)((long)used_ + delta);
+    peak_ = std::max(peak_, used_);
+  }
+
+  void release_group(Group *g) {
+    g->stream->sync();
+    charge(-(long)(g->per_col_bytes() * (size_t)g->cap));
+    g->gk->users -= 1;
+    groups_.erase(std::find_if(groups_.begin(), groups_.end(),
+                               [&](const std::unique_ptr<Group> &p) { return p.get() == g; }));
+    ngroups_ = (int)groups_.size();
+  }
+
+  // Admits r if the budget holds it (engine thread). Returns false to keep
+  // it queued.
+  bool admit(Request *r) {
+    const Cost *cp;
+    {
+      std::lock_guard<std::mutex> lk(mu_);  // costs_ may grow concurrently
+      cp = costs_[r->cost].get();
+    }
+    const Cost &c = *cp;
+    Group *g = nullptr;
+    for (auto &p : groups_)
+      if (p->cost == r->cost && p->reg == r->reg) g = p.get();
+    Gibbs *k = nullptr;
+    for (auto &p : gibbs_)
+      if (p->cost == r->cost && p->reg == r->reg) k = p.get();
+    if (g && g->find_cols(r->S) >= 0) {
+      place(g, r);
+      return true;
+    }
+    Group probe;
+    probe.M = c.M;
+    probe.N = c.N;
+    probe.ldm = pad8(c.M);
+    probe.ldn = pad8(c.N);
+    probe.layout();
+    const size_t pc = probe.per_col_bytes();
+    if (g) {
+      int ncap = g->cap;
+      while (ncap < max_cols_ && g->find_cols(r->S) < 0) {
+        ncap = std::min(max_cols_, std::max(ncap * 2, ncap + r->S));
+        const size_t need = pc * (size_t)(ncap - g->cap);
+        if (!make_room(need, g->gk)) return false;
+        charge((long)need);
+        g->grow(ncap);
+      }
+      if (g->find_cols(r->S) < 0) return false;
+      place(g, r);
+      return true;
+    }
+    const int cap0 = std::min(max_cols_, std::max(kColPage, (r->S + kColPage - 1) / kColPage * kColPage));
+    const size_t kbytes = k ? 0 : dense_kernel_bytes(c.M, c.N);
+    if (!make_room(kbytes + pc * (size_t)cap0, k)) return false;
+    auto ng = std::make_unique<Group>();
+    ng->cost = r->cost;
+    ng->reg = r->reg;
+    ng->C = c.C.get();
+    ng->M = c.M;
+    ng->N = c.N;
+    ng->ldm = pad8(c.M);
+    ng->ldn = pad8(c.N);
+    ng->layout();
+    ng->stream = std::make_unique<Stream>();
+    RWIG_HIP_CHECK(hipEventCreateWithFlags(&ng->ev, hipEventDisableTiming));
+    ng->part.reset(kSumBlocks);
+    ng->scal.reset(1);
+    if (!k) {
+      auto nk = std::make_unique<Gibbs>();
+      nk->cost = r->cost;
+      nk->reg = r->reg;
+      nk->bytes = kbytes;
+      dense_kernel_build(ng->stream->get(), c.C.get(), c.M, c.M, c.N, r->reg, nk->dk);
+      charge((long)kbytes);
+      k = nk.get();
+      gibbs_.push_back(std::move(nk));
+    }
+    k->users += 1;
+    ng->gk = k;
+    charge((long)(pc * (size_t)cap0));
+    ng->grow(cap0);
+    g = ng.get();
+    groups_.push_back(std::move(ng));
+    ngroups_ = (int)groups_.size();
+    place(g, r);
+    return true;
+  }
+
+  void place(Group *g, Request *r) {
+    const int col = g->find_cols(r->S);
+    for (int s = 0; s < r->S; ++s) g->owner[col + s] = r;
+    r->group = g;
+    r->col = col;
+    r->iter = 0;
+    g->init_request(*r);
+    g->running.push_back(r);
+    std::lock_guard<std::mutex> lk(mu_);
+    r->state = kRunning;
+    r->t_admit = Clock::now();
+    qwait_.push_back(ms_between(r->t_submit, r->t_admit));
+    --queued_;
+    ++running_;
+  }
+
+  void finish(Request *r, bool ok, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(mu_);
+    r->state = ok ? kDone : kFailed;
+    r->error = msg;
+    r->t_done = Clock::now();
+    if (ok) {
+      lat_.push_back(ms_between(r->t_submit, r->t_done));
+      ++completed_;
+      t_last_ = r->t_done;
+    } else {
+      ++failed_;
+    }
+  }
+
+  void cycle() {
+    ++cycles_;
+    std::vector<Group *> active;
+    for (auto &g : groups_)
+      if (!g->running.empty()) active.push_back(g.get());
+    for (Group *g : active) {
+      g->last = cycles_;
+      g->gk->last = cycles_;
+      g->upload_lists(0, false);
+      g->absorb(switch_thr_);
+    }
+    for (Group *g : active) {
+      RWIG_HIP_CHECK(hipEventSynchronize(g->ev));
+      std::vector<Request *> keep;
+      for (Request *r : g->running) {
+        if (g->hdone_pin[r->col]) {
+          g->harvest(*r);
+          for (int s = 0; s < r->S; ++s) g->owner[r->col + s] = nullptr;
+          {
+            std::lock_guard<std::mutex> lk(mu_);
+            --running_;
+          }
+          finish(r, true, "");
+          cv_done_.notify_all();
+        } else {
+          keep.push_back(r);
+        }
+      }
+      g->running.swap(keep);
+      if (g->running.empty()) continue;
+      g->upload_lists(1, true);
+      for (int s = 0; s < absorb_every_; ++s) g->iterate();
+      for (Request *r : g->running) r->iter += absorb_every_;
+    }
+  }
+
+  void fail_all(const std::string &msg) {
+    std::vector<Request *> all;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      fatal_ = msg;
+      for (auto &kv : reqs_)
+        if (kv.second->state == kQueued || kv.second->state == kRunning) all.push_back(kv.second.get());
+      incoming_.clear();
+    }
+    for (Request *r : all) finish(r, false, msg);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      queued_ = running_ = 0;
+    }
+    pending_.clear();
+    cv_done_.notify_all();
+  }
+
+  void loop() {
+    (void)hipSetDevice(device_);
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_work_.wait(lk, [&] {
+          return stop_ || !incoming_.empty() || !pending_.empty() || running_ > 0;
+        });
+        if (stop_) break;
+        for (Request *r : incoming_) pending_.push_back(r);
+        incoming_.clear();
+      }
+      try {
+        // admission in arrival order; the first request that does not fit
+        // waits for memory, and so do the ones behind it
+        while (!pending_.empty() && admit(pending_.front())) pending_.pop_front();
+        cycle();
+      } catch (const std::exception &e) {
+        fail_all(e.what());
+        break;
+      }
+    }
+    try {
+      for (auto &g : groups_) g->stream->sync();
+      groups_.clear();
+      gibbs_.clear();
+    } catch (...) {
+    }
+    std::vector<Request *> left;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto &kv : reqs_)
+        if (kv.second->state == kQueued || kv.second->state == kRunning) left.push_back(kv.second.get());
+    }
+    for (Request *r : left) finish(r, false, "serving session destroyed");
+    cv_done_.notify_all();
+  }
+
+  // configuration
+  double budget_;
+  int max_cols_, absorb_every_;
+  double switch_thr_;
+  int device_ = 0;
+  // shared state (mu_)
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::vector<std::unique_ptr<Cost>> costs_;
+  std::unordered_map<long, std::unique_ptr<Request>> reqs_;
+  std::deque<Request *> incoming_;
+  size_t used_ = 0, peak_ = 0, cost_bytes_ = 0;
+  long next_id_ = 1;
+  bool stop_ = false, started_ = false;
+  std::string fatal_;
+  int queued_ = 0, running_ = 0, ngroups_ = 0;
+  long completed_ = 0, failed_ = 0;
+  uint64_t cycles_ = 0;
+  std::vector<double> lat_, qwait_;
+  Clock::time_point t_first_, t_last_;
+  // engine-thread state
+  std::deque<Request *> pending_;
+  std::vector<std::unique_ptr<Group>> groups_;
+  std::vector<std::unique_ptr<Gibbs>> gibbs_;
+  std::thread th_;
+};
+
+}  // namespace rwig
+
+using namespace rwig;
+
+namespace {
+
+ServeEngine *engine_of(void *h) {
+  RWIG_REQUIRE(h != nullptr, "null serving session");
+  return static_cast<ServeEngine *>(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rwig_serve_create(double hbm_budget_bytes, int max_group_columns, int absorb_every,
+                      double switch_thr, void **session) {
+  return guarded("rwig_serve_create", [&] {
+    RWIG_REQUIRE(session != nullptr, "null session pointer");
+    RWIG_REQUIRE(hbm_budget_bytes > 0, "HBM budget must be positive");
+    RWIG_REQUIRE(max_group_columns >= 1 && max_group_columns <= 4096,
+                 "max_group_columns must be in [1, 4096]");
+    RWIG_REQUIRE(absorb_every >= 1, "absorb_every must be positive");
+    ensure_device();
+    *session = new ServeEngine(hbm_budget_bytes, max_group_columns, absorb_every, switch_thr);
+  });
+}
+
+int rwig_serve_destroy(void *session) {
+  return guarded("rwig_serve_destroy", [&] { delete engine_of(session); });
+}
+
+int rwig_serve_register_cost(void *session, const double *C, int M, int N, int *cost_id) {
+  return guarded("rwig_serve_register_cost", [&] {
+    RWIG_REQUIRE(C && cost_id && M > 0 && N > 0, "invalid argument");
+    *cost_id = engine_of(session)->register_cost(C, M, N);
+  });
+}
+
+int rwig_serve_submit_sinkhorn(void *session, int cost_id, double reg, const double *a, int M,
+                               const double *b, int N, int max_iter, double zero_tol,
+                               long *request_id) {
+  return guarded("rwig_serve_submit_sinkhorn", [&] {
+    RWIG_REQUIRE(a && b && request_id && M > 0 && N > 0, "invalid argument");
+    RWIG_REQUIRE(reg > 0.0 && max_iter >= 0, "reg must be positive, max_iter non-negative");
+    for (int i = 0; i < M; ++i) RWIG_REQUIRE(a[i] > 0.0, "entries of a must be positive");
+    for (int j = 0; j < N; ++j) RWIG_REQUIRE(b[j] > 0.0, "entries of b must be positive");
+    auto r = std::make_unique<Request>();
+    r->kind = kSinkhorn;
+    r->cost = cost_id;
+    r->reg = reg;
+    r->S = 1;
+    r->max_iter = max_iter;
+    r->tol = zero_tol;
+    r->in1.assign(a, a + M);
+    r->in2.assign(b, b + N);
+    *request_id = engine_of(session)->submit(std::move(r));
+  });
+}
+
+int rwig_serve_submit_barycenter(void *session, int cost_id, double reg, const double *A, int M,
+                                 int S, const double *w, int max_iter, double zero_tol,
+                                 long *request_id) {
+  return guarded("rwig_serve_submit_barycenter", [&] {
+    RWIG_REQUIRE(A && w && request_id && M > 0 && S >= 1, "invalid argument");
+    RWIG_REQUIRE(reg > 0.0 && max_iter >= 0, "reg must be positive, max_iter non-negative");
+    for (long k = 0; k < (long)M * S; ++k)
+      RWIG_REQUIRE(A[k] > 0.0, "entries of A must be positive");
+    auto r = std::make_unique<Request>();
+    r->kind = kBarycenter;
+    r->cost = cost_id;
+    r->reg = reg;
+    r->S = S;
+    r->max_iter = max_iter;
+    r->tol = zero_tol;
+    r->in1.assign(A, A + (long)M * S);
+    r->in2.assign(w, w + S);
+    *request_id = engine_of(session)->submit(std::move(r));
+  });
+}
+
+int rwig_serve_wait(void *session, long request_id, double timeout_s, int *state) {
+  return guarded("rwig_serve_wait", [&] {
+    const int s = engine_of(session)->wait(request_id, timeout_s);
+    if (state) *state = s;
+  });
+}
+
+int rwig_serve_result(void *session, long request_id, double *x, double *y, double *loss,
+                      int *iter, double *err, double *latency_ms) {
+  int code = RWIG_OK;
+  std::string msg;
+  const int rc = guarded("rwig_serve_result", [&] {
+    std::unique_ptr<Request> r = engine_of(session)->take(request_id);
+    if (r->state == kFailed) {
+      code = RWIG_ERR_HIP;
+      msg = "request failed: " + r->error;
+      return;
+    }
+    if (x) std::copy(r->out1.begin(), r->out1.end(), x);
+    if (y && r->kind == kSinkhorn) std::copy(r->out2.begin(), r->out2.end(), y);
+    if (loss) *loss = r->loss;
+    if (iter) *iter = r->iter;
+    if (err) *err = r->err;
+    if (latency_ms) *latency_ms = ms_between(r->t_submit, r->t_done);
+  });
+  if (rc != RWIG_OK) return rc;
+  if (code != RWIG_OK) {
+    set_last_error("rwig_serve_result: " + msg);
+    return code;
+  }
+  return RWIG_OK;
+}
+
+int rwig_serve_stats(void *session, double *out, int n_out) {
+  return guarded("rwig_serve_stats", [&] {
+    RWIG_REQUIRE(out && n_out >= 0, "invalid argument");
+    engine_of(session)->stats(out, n_out);
+  });
+}
+
+int rwig_serve_reset_stats(void *session) {
+  return guarded("rwig_serve_reset_stats", [&] { engine_of(session)->reset_stats(); });
+}
+
+}  // extern "C"

@@ -15,7 +15,7 @@
 #include <random>
 
 #include "gemm.hpp"
-#include "kernels_fp64.hpp"
+#include "kernels.hpp"
 
 namespace rwig {
 

@@ -38,3 +38,36 @@ def c2(n=4096, B=1024, frac=0.05, seed=1002):
         supports.append(s)
         weights.append(w / w.sum())
     return np.asfortranarray(C), a, supports, weights, 0.01
+
+
+def c3(grid=64, S=8, seed=1003):
+    """Barycenter with unrolled gradient: S Gaussian-mixture histograms on a
+    grid x grid lattice of [0,1]^2 (3 isotropic components each, means in
+    [0.2, 0.8]^2, sigma in [0.03, 0.1]), weights ~ Dirichlet(1), squared
+    Euclidean cost normalised to max 1, eps 0.004, L = 200 iterations.
+
+    Returns (A (n x S, Fortran order), C (n x n, Fortran order), w, b_ext, eps,
+    L); b_ext is a further mixture histogram, the target of the loss.
+    """
+    rng = np.random.default_rng(seed)
+    t = (np.arange(grid) + 0.5) / grid
+    X = np.stack(np.meshgrid(t, t, indexing="ij"), -1).reshape(-1, 2)
+    n = X.shape[0]
+
+    def mixture():
+        h = np.zeros(n)
+        for _ in range(3):
+            mu = rng.uniform(0.2, 0.8, 2)
+            sig = rng.uniform(0.03, 0.1)
+            h += np.exp(-((X - mu) ** 2).sum(1) / (2.0 * sig * sig))
+        h = np.maximum(h, 1e-300)
+        return h / h.sum()
+
+    A = np.stack([mixture() for _ in range(S)], 1)
+    w = rng.dirichlet(np.ones(S))
+    b_ext = mixture()
+    sq = (X * X).sum(1)
+    C = np.maximum(sq[:, None] + sq[None, :] - 2.0 * X @ X.T, 0.0)
+    np.fill_diagonal(C, 0.0)
+    C /= C.max()
+    return np.asfortranarray(A), np.asfortranarray(C), w, b_ext, 0.004, 200

@@ -1,0 +1,150 @@
+"""GPU tests of the checkpointed log barycenter (rwig_barycenter_log_ck).
+
+Bars:
+- FP64, reference seed: equal to the plain FP64 log barycenter and to the
+  oracle to 1e-10 relative, for every checkpoint interval (the recomputation
+  replays the forward's own kernels, so the history it rebuilds is the
+  forward's).
+- FP64, exact seeds: gradients within 1e-5 relative of central finite
+  differences (h = 1e-6) of the loss each seed differentiates.
+- FP32 (iterates and soft-min passes; FP64 seed, loss and accumulation):
+  within TOL_FP32_* of the FP64 result on the same inputs.
+"""
+import numpy as np
+import pytest
+
+import rwig_amd
+from oracle import oracle as O
+from rwig_amd.workloads import c3
+from tests.test_gpu_plain import bary_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_FD = 1e-5
+H_FD = 1e-6
+TOL_FP32_B = 1e-5
+TOL_FP32_GRAD = 1e-4
+
+
+def ck(A, C, w, bext, reg, L, **kw):
+    kw.setdefault("zero_tol", 0.0)
+    return rwig_amd.barycenter_log_ck(A, C, w, bext, reg=reg, max_iter=L, **kw)
+
+
+@pytest.mark.parametrize("seed,M,N,S,reg", [(50, 40, 50, 3, 0.05), (51, 120, 100, 5, 0.02)])
+@pytest.mark.parametrize("every", [1, 7, 20, 1000])
+def test_fp64_reference_seed_matches_plain_and_oracle(seed, M, N, S, reg, every):
+    A, C, w, bext = bary_problem(seed, M, N, S)
+    r = rwig_amd.barycenter_log_ck(A, C, w, bext, reg=reg, with_grad=True, max_iter=1000,
+                                   zero_tol=1e-6, checkpoint_every=every)
+    p = rwig_amd.barycenter(A, C, w, bext, {"reg": reg, "method": "log", "with_grad": True})
+    o = O.barycenter_log(A, C, w, bext, reg, withgrad=True)
+    assert r["iter"] == p["iter"] == o["iter"]
+    assert r["return_status"] == p["return_status"]
+    for k in ["b", "F", "G", "grad_A", "grad_w", "loss"]:
+        assert rel(r[k], p[k]) < 1e-12, k
+        assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+def test_no_grad_and_zero_iterations():
+    A, C, w, bext = bary_problem(52, 30, 30, 2)
+    r0 = rwig_amd.barycenter_log_ck(A, C, w, bext, reg=0.1, with_grad=True, max_iter=0)
+    p0 = rwig_amd.barycenter(A, C, w, bext, {"reg": 0.1, "method": "log", "with_grad": True,
+                                             "max_iter": 0})
+    assert r0["iter"] == p0["iter"] == 0
+    assert np.all(r0["grad_A"] == 0) and np.all(r0["grad_w"] == 0)
+    np.testing.assert_allclose(r0["b"], np.full(30, 1 / 30), rtol=1e-15)
+    assert r0["loss"] == pytest.approx(((1 / 30 - bext) ** 2).sum(), rel=1e-14)
+    assert rel(r0["loss"], p0["loss"]) < 1e-14
+    r1 = rwig_amd.barycenter_log_ck(A, C, w, reg=0.1, max_iter=50)
+    p = rwig_amd.barycenter(A, C, w, None, {"reg": 0.1, "method": "log", "max_iter": 50})
+    assert r1["iter"] == p["iter"]
+    assert rel(r1["b"], p["b"]) < 1e-12
+
+
+def test_invalid_arguments_raise():
+    A, C, w, bext = bary_problem(53, 10, 10, 2)
+    with pytest.raises(rwig_amd.RwigError):
+        rwig_amd.barycenter_log_ck(A, C, w, bext, reg=0.1, with_grad=True, checkpoint_every=0)
+    with pytest.raises(ValueError):
+        rwig_amd.barycenter_log_ck(A, C, w, bext, reg=0.1, with_grad=True, seed="other")
+
+
+def fd_check(A, C, w, bext, reg, L, seed, coords, every=20):
+    """Central differences of the seed's loss over A[coords] and all of w."""
+    g = ck(A, C, w, bext, reg, L, with_grad=True, seed=seed, checkpoint_every=every)
+
+    def loss(A_, w_):
+        if seed == "normalised":
+            b = ck(A_, C, w_, None, reg, L)["b"]
+            return float(((b - bext) ** 2).sum())
+        return ck(A_, C, w_, bext, reg, L, with_grad=True, seed=seed)["loss"]
+
+    assert abs(loss(A, w) - g["loss"]) <= 1e-14 * max(1.0, abs(g["loss"]))
+    fdA, an_A = [], []
+    for (i, s) in coords:
+        Ap, Am = A.copy(order="F"), A.copy(order="F")
+        Ap[i, s] += H_FD
+        Am[i, s] -= H_FD
+        fdA.append((loss(Ap, w) - loss(Am, w)) / (2 * H_FD))
+        an_A.append(g["grad_A"][i, s])
+    fdw = []
+    for s in range(len(w)):
+        wp, wm = w.copy(), w.copy()
+        wp[s] += H_FD
+        wm[s] -= H_FD
+        fdw.append((loss(A, wp) - loss(A, wm)) / (2 * H_FD))
+    fd = np.array(fdA + fdw)
+    an = np.array(an_A + list(g["grad_w"]))
+    err = np.linalg.norm(fd - an) / np.linalg.norm(fd)
+    print(f"FD check seed={seed} L={L}: |fd - adjoint| / |fd| = {err:.2e}")
+    return err
+
+
+@pytest.mark.parametrize("seed", ["unnormalised", "normalised"])
+def test_fp64_gradient_matches_finite_differences(seed):
+    A, C, w, bext = bary_problem(54, 64, 64, 4)
+    rng = np.random.default_rng(5)
+    coords = [(int(rng.integers(64)), int(rng.integers(4))) for _ in range(12)]
+    assert fd_check(A, C, w, bext, 0.02, 60, seed, coords, every=7) < TOL_FD
+
+
+def c3_inputs():
+    A, C, w, bext, eps, L = c3()
+    return A, C, w, bext, eps, L
+
+
+def test_c3_fp64_gradient_matches_finite_differences():
+    # Gaussian-mixture barycenter on the 64 x 64 grid, S = 8, eps 0.004,
+    # L = 200, checkpoints every 20; A coordinates in the bulk of each
+    # histogram (A > 1e-4) so that A +- h stays positive
+    A, C, w, bext, eps, L = c3_inputs()
+    rng = np.random.default_rng(6)
+    coords = []
+    for s in range(A.shape[1]):
+        bulk = np.flatnonzero(A[:, s] > 1e-4)
+        coords.append((int(rng.choice(bulk)), s))
+    assert fd_check(A, C, w, bext, eps, L, "unnormalised", coords, every=20) < TOL_FD
+
+
+def test_c3_fp32_against_fp64():
+    # FP32 iterates resolve the gradient of entries of A down to about 1e-12
+    # of the histogram's mass; below that the adjoint's values sit under FP32
+    # resolution relative to the problem's scale (measured: relative error
+    # 2.5e-5 for A in [1e-12, 1e-6), 2e-6 above; O(1) below 1e-12)
+    A, C, w, bext, eps, L = c3_inputs()
+    r64 = ck(A, C, w, bext, eps, L, with_grad=True, seed="unnormalised", checkpoint_every=20)
+    r32 = ck(A, C, w, bext, eps, L, with_grad=True, seed="unnormalised", checkpoint_every=20,
+             precision="fp32")
+    assert r32["iter"] == r64["iter"] == L
+    eb = rel(r32["b"], r64["b"])
+    big = A >= 1e-12
+    d = r32["grad_A"] - r64["grad_A"]
+    egA = np.linalg.norm(d[big]) / np.linalg.norm(r64["grad_A"][big])
+    egw = np.linalg.norm(r32["grad_w"] - r64["grad_w"]) / np.linalg.norm(r64["grad_w"])
+    el = abs(r32["loss"] - r64["loss"]) / abs(r64["loss"])
+    print(f"C3 FP32 vs FP64: b {eb:.2e}  grad_A(A>=1e-12) {egA:.2e}  grad_w {egw:.2e}  "
+          f"loss {el:.2e}")
+    assert eb < TOL_FP32_B and el < TOL_FP32_B
+    assert egA < TOL_FP32_GRAD and egw < TOL_FP32_GRAD

@@ -1,0 +1,182 @@
+"""GPU tests of the serving engine (rwig_serve_*).
+
+Bars:
+- every result meets its stopping rule when checked on the host in FP64
+  (L1 marginal error of the returned potentials' plan <= zero_tol);
+- Sinkhorn losses and barycenters agree with the FP64 oracle run to a tight
+  tolerance within TOL_LOSS / TOL_BARY (the engine stops at L1 error 1e-6);
+- a request's result is bit-identical alone and in a mix of other requests
+  (same and other cost matrices, submitted from two threads while it runs);
+- the HBM footprint never exceeds the session's budget, and requests that do
+  not fit wait in the queue and still complete.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rwig_amd import RwigError
+from rwig_amd.serve import DONE, ServeSession
+
+pytestmark = pytest.mark.gpu
+
+TOL_LOSS = 1e-5
+TOL_BARY = 1e-5
+
+
+def cloud(rng, n):
+    X = rng.random((n, 2))
+    sq = (X * X).sum(1)
+    C = np.maximum(sq[:, None] + sq[None] - 2.0 * X @ X.T, 0.0)
+    np.fill_diagonal(C, 0.0)
+    return C
+
+
+def marginal_err(C, eps, f, g, a, b):
+    P = np.exp((f[:, None] + g[None] - C) / eps)
+    return np.abs(P.sum(1) - a).sum() + np.abs(P.sum(0) - b).sum()
+
+
+@pytest.fixture(scope="module")
+def clouds():
+    rng = np.random.default_rng(2005)
+    return [cloud(rng, 200), cloud(rng, 700), cloud(rng, 64)]
+
+
+def test_sinkhorn_accuracy_against_oracle(clouds):
+    rng = np.random.default_rng(1)
+    with ServeSession(hbm_budget_bytes=2 << 30) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        jobs = []
+        for k, eps in [(0, 0.1), (0, 0.01), (1, 0.1), (1, 0.01), (2, 0.01)]:
+            n = clouds[k].shape[0]
+            a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+            jobs.append((k, eps, a, b, s.submit_sinkhorn(ids[k], a, b, eps, zero_tol=1e-6)))
+        for k, eps, a, b, rid in jobs:
+            r = s.result(rid)
+            C = clouds[k]
+            assert r["err"] <= 1e-6
+            e_host = marginal_err(C, eps, r["f"], r["g"], a, b)
+            assert abs(e_host - r["err"]) <= 1e-9, (e_host, r["err"])
+            o = O.sinkhorn_log(a, b, C, eps, maxiter=5000, zerotol=1e-12)
+            dl = abs(r["loss"] - o["loss"]) / abs(o["loss"])
+            print(f"n={C.shape[0]} eps={eps}: iter {r['iter']} err {r['err']:.2e} "
+                  f"loss rel diff {dl:.2e}")
+            assert dl < TOL_LOSS
+
+
+def test_barycenter_accuracy_against_oracle(clouds):
+    rng = np.random.default_rng(2)
+    with ServeSession(hbm_budget_bytes=2 << 30) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        for k, S, eps in [(0, 2, 0.1), (1, 4, 0.1), (0, 8, 0.05)]:
+            n = clouds[k].shape[0]
+            A = rng.dirichlet(np.ones(n), size=S).T.copy()
+            w = rng.dirichlet(np.ones(S))
+            r = s.result(s.submit_barycenter(ids[k], A, w, eps, zero_tol=1e-6))
+            o = O.barycenter_parallel(A, clouds[k], w, None, eps, maxiter=20000, zerotol=1e-13)
+            d = np.abs(r["b"] - o["b"]).sum()
+            print(f"barycenter n={n} S={S} eps={eps}: iter {r['iter']} err {r['err']:.2e} "
+                  f"|b - b_oracle|_1 {d:.2e}")
+            assert r["err"] <= 1e-6
+            assert abs(r["b"].sum() - 1.0) < 1e-12
+            assert d < TOL_BARY
+
+
+def test_alone_equals_mix(clouds):
+    rng = np.random.default_rng(3)
+    n0, n1 = clouds[0].shape[0], clouds[1].shape[0]
+    a, b = rng.dirichlet(np.ones(n0)), rng.dirichlet(np.ones(n0))
+    A = rng.dirichlet(np.ones(n1), size=4).T.copy()
+    w = rng.dirichlet(np.ones(4))
+    with ServeSession(hbm_budget_bytes=2 << 30) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        alone_s = s.result(s.submit_sinkhorn(ids[0], a, b, 0.01))
+        alone_b = s.result(s.submit_barycenter(ids[1], A, w, 0.1))
+
+    def flood(sess, seed, out):
+        r = np.random.default_rng(seed)
+        for k in range(12):
+            c = int(r.integers(3))
+            n = clouds[c].shape[0]
+            eps = [0.1, 0.01][k % 2]
+            if k % 4 == 3:
+                S = int(r.choice([2, 4, 8]))
+                out.append(sess.submit_barycenter(ids[c], r.dirichlet(np.ones(n), size=S).T.copy(),
+                                                  r.dirichlet(np.ones(S)), eps))
+            else:
+                out.append(sess.submit_sinkhorn(ids[c], r.dirichlet(np.ones(n)),
+                                                r.dirichlet(np.ones(n)), eps))
+
+    with ServeSession(hbm_budget_bytes=2 << 30) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        others = []
+        flood(s, 10, others)  # already running when the probes arrive
+        rs = s.submit_sinkhorn(ids[0], a, b, 0.01)
+        t = threading.Thread(target=flood, args=(s, 11, others))
+        t.start()
+        rb = s.submit_barycenter(ids[1], A, w, 0.1)
+        t.join()
+        mix_s, mix_b = s.result(rs), s.result(rb)
+        for rid in others:
+            assert s.wait(rid) == DONE
+            s.result(rid)
+        st = s.stats()
+    assert mix_s["iter"] == alone_s["iter"] and mix_b["iter"] == alone_b["iter"]
+    assert np.array_equal(mix_s["f"], alone_s["f"]) and np.array_equal(mix_s["g"], alone_s["g"])
+    assert mix_s["loss"] == alone_s["loss"] and mix_s["err"] == alone_s["err"]
+    assert np.array_equal(mix_b["b"], alone_b["b"]) and mix_b["err"] == alone_b["err"]
+    assert st["completed"] == 26 and st["failed"] == 0
+
+
+def test_admission_by_hbm_budget(clouds):
+    # budget for the three cost matrices and not all three groups at once:
+    # requests queue until an idle group is released, within the budget
+    rng = np.random.default_rng(4)
+    cost_bytes = sum(C.nbytes for C in clouds)
+    budget = cost_bytes + int(12.5 * 2**20)  # the 700-point group fits, not with the others
+    with ServeSession(hbm_budget_bytes=budget, max_group_columns=64) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        jobs = []
+        for k in range(9):
+            c = k % 3
+            n = clouds[c].shape[0]
+            a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+            jobs.append((c, a, b, s.submit_sinkhorn(ids[c], a, b, 0.1)))
+        for c, a, b, rid in jobs:
+            r = s.result(rid)
+            assert r["err"] <= 1e-6
+            assert marginal_err(clouds[c], 0.1, r["f"], r["g"], a, b) <= 1e-6
+        st = s.stats()
+    print("admission stats:", {k: st[k] for k in ("peak_bytes", "budget_bytes", "mean_queue_ms",
+                                                   "p50_ms", "p99_ms")})
+    assert st["completed"] == 9
+    assert st["peak_bytes"] <= budget
+    assert st["mean_queue_ms"] > 0
+
+
+def test_request_larger_than_budget_is_rejected(clouds):
+    with ServeSession(hbm_budget_bytes=clouds[1].nbytes + (1 << 20)) as s:
+        cid = s.register_cost(clouds[1])
+        n = clouds[1].shape[0]
+        with pytest.raises(RwigError):
+            s.submit_sinkhorn(cid, np.full(n, 1.0 / n), np.full(n, 1.0 / n), 0.1)
+
+
+def test_max_iter_and_invalid_input(clouds):
+    rng = np.random.default_rng(5)
+    n = clouds[0].shape[0]
+    with ServeSession(hbm_budget_bytes=1 << 30, absorb_every=10) as s:
+        cid = s.register_cost(clouds[0])
+        a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+        r = s.result(s.submit_sinkhorn(cid, a, b, 0.01, max_iter=30, zero_tol=1e-12))
+        assert r["iter"] == 30 and r["err"] > 1e-12
+        r0 = s.result(s.submit_sinkhorn(cid, a, b, 0.01, max_iter=0))
+        assert r0["iter"] == 0
+        bad = a.copy()
+        bad[3] = 0.0
+        with pytest.raises(RwigError):
+            s.submit_sinkhorn(cid, bad, b, 0.1)
+        with pytest.raises(RwigError):
+            s.submit_sinkhorn(cid + 5, a, b, 0.1)

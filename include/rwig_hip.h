@@ -249,9 +249,17 @@ int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
  * each absorption, or max_iter reached (iteration counts are multiples of
  * absorb_every). A request's result does not depend on the other requests
  * in flight. Entries of a, b and A must be positive. Calls are thread-safe. */
+#define RWIG_SERVE_EAGER 1 /* flags: launch every kernel of a cycle directly
+                             instead of replaying recorded HIP graphs */
 int rwig_serve_create(double hbm_budget_bytes, int max_group_columns,
-                      int absorb_every, double switch_thr, void **session);
+                      int absorb_every, double switch_thr, int flags,
+                      void **session);
 int rwig_serve_destroy(void *session);
+/* HBM bytes the session charges for a (cost, reg) pair of an M x N cost: its
+ * cached Gibbs kernel, and its group's buffers at `columns` columns (groups
+ * start at 32 columns and double as requests arrive). */
+int rwig_serve_footprint(int M, int N, int columns, double *kernel_bytes,
+                         double *group_bytes);
 /* C: host, column-major M x N, copied to the device. */
 int rwig_serve_register_cost(void *session, const double *C, int M, int N,
                              int *cost_id);

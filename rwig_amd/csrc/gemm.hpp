@@ -14,6 +14,8 @@
 // op(m, n, value) so callers can fuse their elementwise update.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace rwig {
@@ -145,10 +147,80 @@ struct TileLoader {
   }
 };
 
+// Deferred epilogue, with an optional split of the k range over blockIdx.z
+// (skinny products: few row tiles, long k). With a workspace, each split
+// writes its partial tile there and a second kernel adds the partials in
+// split order and runs the epilogue with more workgroups than the product
+// has (the epilogue's loads are latency-bound at one workgroup per tile).
+// Without one (ks = 1), the epilogue runs in the product's kernel.
+template <typename T> struct SplitK {
+  int ks = 1;
+  typename Mfma<T>::out_t *work = nullptr;  // ks x tiles x (per-thread tile) x 256
+};
+
+// Accumulator elements per thread of one workgroup tile.
+template <typename T> constexpr int gemm_acc_per_thread() {
+  return (GemmCfg<T>::BM / 2 / 16) * (GemmCfg<T>::BN / 2 / 16) * 4;
+}
+
+// k splits for an M x K operand: enough workgroups along M x splits to cover
+// the chip when M alone gives few tiles, at least 4 k-steps per split. A
+// function of (M, K) only, so that the arithmetic of a column does not depend
+// on how many columns the product has.
+template <typename T> inline int splitk_for(int M, int K) {
+  const int tiles = (M + GemmCfg<T>::BM - 1) / GemmCfg<T>::BM;
+  const int steps = (K + GemmCfg<T>::BK - 1) / GemmCfg<T>::BK;
+  int ks = 256 / tiles;
+  ks = std::min(ks, 8);
+  ks = std::min(ks, steps / 4);
+  return std::max(ks, 1);
+}
+
+// Workspace elements of a split product with M rows, up to N columns and ks
+// splits.
+template <typename T> inline size_t splitk_work_elems(int M, int N, int ks) {
+  const size_t tiles = (size_t)((M + GemmCfg<T>::BM - 1) / GemmCfg<T>::BM) *
+                       ((N + GemmCfg<T>::BN - 1) / GemmCfg<T>::BN);
+  return (size_t)ks * tiles * gemm_acc_per_thread<T>() * 256;
+}
+
+// Epilogue of one workgroup tile: op(m, n or nmap[n], value) for the tile's
+// elements inside M x N; accumulator lane layout: column lane % 16, row
+// Mfma::row(lane, r).
+template <typename T, class Op>
+__device__ __forceinline__ void gemm_epilogue(
+    int M, int N, int m0, int n0, const int *__restrict__ nmap,
+    const typename Mfma<T>::acc_t (&acc)[GemmCfg<T>::BM / 32][GemmCfg<T>::BN / 32], Op &op) {
+  using MF = Mfma<T>;
+  constexpr int WM = GemmCfg<T>::BM / 2, WN = GemmCfg<T>::BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv & 1, wn = wv >> 1, lr = lane & 15;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WN + j * 16 + lr;
+      if (n >= N) continue;
+      const int nn = nmap ? nmap[n] : n;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WM + i * 16 + MF::row(lane, r);
+        if (m < M) op(m, nn, acc[i][j][r]);
+      }
+    }
+}
+
 template <typename T, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
-          long ldb, const int *__restrict__ nmap, Op op) {
+          long ldb, const int *__restrict__ nmap, const int *__restrict__ ncount, SplitK<T> sk,
+          Op op) {
+  if (ncount) {  // column count read on the device (the grid covers an upper bound)
+    const int nc = *ncount;
+    if ((int)blockIdx.y * GemmCfg<T>::BN >= nc) return;
+    N = nc;
+  }
   using MF = Mfma<T>;
   constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN, BK = GemmCfg<T>::BK;
   constexpr int LDL = BK + GemmCfg<T>::PAD;
@@ -171,17 +243,21 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 
   TileLoader<T, BM, BK, 256> la;
   TileLoader<T, BN, BK, 256> lb;
+  // k range of this split (multiples of BK)
+  const int steps = (K + BK - 1) / BK;
+  const int per = (steps + sk.ks - 1) / sk.ks;
+  const int kbeg = blockIdx.z * per * BK, kend = min(K, ((int)blockIdx.z + 1) * per * BK);
   la.init(A, lda, M, m0, nullptr);
   lb.init(B, ldb, N, n0, nmap);
-  la.load(K, 0);
-  lb.load(K, 0);
+  la.load(K, kbeg);
+  lb.load(K, kbeg);
   la.store(As[0], LDL);
   lb.store(Bs[0], LDL);
   __syncthreads();
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
   int cur = 0;
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    const bool more = k0 + BK < K;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
     if (more) {
       la.load(K, k0 + BK);
       lb.load(K, k0 + BK);
@@ -206,37 +282,91 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
     __syncthreads();
     cur ^= 1;
   }
-  // accumulator lane layout: column lr, row MF::row(lane, r)
+  if (sk.work) {
+    // partial tile of this split; the sum and the epilogue follow in
+    // k_gemm_splitk_epilogue (the kernel boundary orders the writes)
+    constexpr int NE = TM * TN * 4;
+    const long ntile = (long)gridDim.x * gridDim.y;
+    const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    typename MF::out_t *w = sk.work + ((long)blockIdx.z * ntile + tile) * NE * 256 + threadIdx.x;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn * WN + j * 16 + lr;
-      if (n >= N) continue;
-      const int nn = nmap ? nmap[n] : n;
+      for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * WM + i * 16 + MF::row(lane, r);
-        if (m < M) op(m, nn, acc[i][j][r]);
-      }
-    }
+        for (int r = 0; r < 4; ++r) w[((i * TN + j) * 4 + r) * 256] = acc[i][j][r];
+    return;
+  }
+  gemm_epilogue<T>(M, N, m0, n0, nmap, acc, op);
+}
+
+// Sum of the split partials of each tile, in split order, then the epilogue.
+// Same tile and thread layout as the product, with one 16 x 16 fragment
+// (i, j) = blockIdx.z per workgroup: TM x TN times the product's workgroups,
+// so that the epilogue's loads have the occupancy to hide their latency.
+template <typename T, class Op>
+__global__ void __launch_bounds__(256)
+k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
+                       const int *__restrict__ ncount, int ks,
+                       const typename Mfma<T>::out_t *__restrict__ work, Op op) {
+  using MF = Mfma<T>;
+  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16, NE = TM * TN * 4;
+  if (ncount) {
+    const int nc = *ncount;
+    if ((int)blockIdx.y * BN >= nc) return;
+    N = nc;
+  }
+  const int i = blockIdx.z / TN, j = blockIdx.z % TN;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv & 1, wn = wv >> 1;
+  const int n = blockIdx.y * BN + wn * WN + j * 16 + (lane & 15);
+  if (n >= N) return;
+  const long ntile = (long)gridDim.x * gridDim.y;
+  const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  const long zs = ntile * NE * 256;
+  const typename MF::out_t *w0 = work + tile * NE * 256 + ((i * TN + j) * 4) * 256 + threadIdx.x;
+  typename MF::out_t v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    typename MF::out_t x = w0[r * 256];
+    for (int z = 1; z < ks; ++z) x += w0[z * zs + r * 256];
+    v[r] = x;
+  }
+  const int nn = nmap ? nmap[n] : n;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = blockIdx.x * BM + wm * WM + i * 16 + MF::row(lane, r);
+    if (m < M) op(m, nn, v[r]);
+  }
 }
 
 // nmap (optional, N entries): GEMM column n reads column nmap[n] of B and is
 // reported to the epilogue as column nmap[n] -- a product over a subset of
-// stored columns without moving them.
+// stored columns without moving them. ncount (optional, device): the number
+// of columns actually used, read by the kernel (N is then an upper bound that
+// sizes the grid), so that a launch can be recorded once in a graph and
+// replayed over a changing column set.
 template <typename T, class Op>
 void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
-             Op op, const int *nmap = nullptr) {
+             Op op, const int *nmap = nullptr, const int *ncount = nullptr,
+             const SplitK<T> &sk = SplitK<T>()) {
   constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
   RWIG_REQUIRE(M > 0 && N >= 0 && K > 0, "GEMM dimensions must be positive");
   RWIG_REQUIRE(lda % (16 / sizeof(T)) == 0 && ldb % (16 / sizeof(T)) == 0,
                "GEMM leading dimensions must be multiples of 16 bytes");
+  RWIG_REQUIRE(sk.ks >= 1 && (sk.ks == 1 || sk.work), "split-k product needs its workspace");
   if (N == 0) return;
-  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN);
+  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, sk.ks);
   hipLaunchKernelGGL((k_gemm_tn<T, Op>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, nmap,
-                     op);
+                     ncount, sk, op);
   check_launch();
+  if (sk.work) {
+    constexpr int NF = (GemmCfg<T>::BM / 32) * (GemmCfg<T>::BN / 32);
+    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, Op>), dim3(grid.x, grid.y, NF), dim3(256), 0,
+                       st, M, N, nmap, ncount, sk.ks, sk.work, op);
+    check_launch();
+  }
 }
 
 }  // namespace rwig

@@ -53,6 +53,27 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// f(idx, row) for idx < cnt[which] (a count read on the device) and row <
+// rows: the grid covers an upper bound of the count, so the launch can be
+// recorded once in a graph and replayed over changing column lists.
+template <class F>
+__global__ void __launch_bounds__(kTB) k_cols(long rows, const int *cnt, int which, F f) {
+  const long n = rows * (long)cnt[which];
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x)
+    f((int)(k / rows), k % rows);
+}
+
+template <class F>
+void launch_cols(hipStream_t st, long rows, int bound, const int *cnt, int which, F f) {
+  hipLaunchKernelGGL(k_cols<F>, dim3(grid_for(rows * bound, kTB, 8192)), dim3(kTB), 0, st, rows,
+                     cnt, which, f);
+  check_launch();
+}
+
+// indices into the device list header
+constexpr int kNact = 0, kNreq = 1, kNbar = 2;
+
 // Stopping test of each running request at an absorption, one block per
 // request: L1 marginal error of the absorbed plans, sum over the request's
 // columns; for a barycenter the common target is re-estimated from the exact
@@ -65,8 +86,9 @@ k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
             const int *__restrict__ maxit, const double *__restrict__ tol,
             const double *__restrict__ acol, const double *__restrict__ r0, double *bcol,
             const double *__restrict__ c0, const double *__restrict__ G, double switch_thr,
-            int *done, int *mode, double *errs) {
+            int *done, int *mode, double *errs, const int *cnt) {
   __shared__ double sh[256 / kWave];
+  if ((int)blockIdx.x >= cnt[kNreq]) return;
   const int base = areq[blockIdx.x];
   const int S = ns[base];
   double e = 0.0;
@@ -124,6 +146,7 @@ struct Request {
   double reg = 0.0, tol = 0.0;
   std::vector<double> in1, in2;  // Sinkhorn: a (M), b (N); barycenter: A (M x S), w (S)
   std::vector<double> out1, out2;  // Sinkhorn: f (M), g (N); barycenter: b (N)
+  std::vector<double> r0, c0;      // Sinkhorn: marginals of the final plan (loss)
   double loss = 0.0, err = 0.0;
   int iter = 0;
   int state = kQueued;
@@ -145,12 +168,15 @@ struct Group {
   int cost = -1;
   double reg = 0.0;
   Gibbs *gk = nullptr;
-  const double *C = nullptr;
   int M = 0, N = 0;
   long ldm = 0, ldn = 0;
   int cap = 0;
   std::unique_ptr<Stream> stream;  // created with the group
   hipEvent_t ev = nullptr;
+  bool graphs = true;  // replay each phase of a cycle as a HIP graph
+  int absorb_every = 20;
+  double switch_thr = 1e-2;
+  hipGraphExec_t gx[2] = {nullptr, nullptr};  // absorption, iterations
   uint64_t last = 0;
   std::vector<Request *> owner;  // column -> request occupying it
   std::vector<Request *> running;
@@ -159,18 +185,19 @@ struct Group {
   Slab bcol, G, c0, lv, be, sb, Zq, X, Bd;   // target side (N rows)
   Slab kind, base, ns, done, mode, maxit;    // ints per column
   Slab wcol, tol, errs;                      // doubles per column
-  // column and request lists: act | areq | riter | abar, 4 x cap ints
-  Slab lists;
-  int *hl[2] = {nullptr, nullptr};  // pinned host copies, one per upload phase
-  DBuf<double> part, scal;          // loss reduction partials and result
+  // device lists: header (nact, nreq, nbar, -) | act | areq | riter | abar
+  DBuf<int> lists;
+  int *hl[2] = {nullptr, nullptr};  // pinned host copies, one per cycle phase
   int *hdone_pin = nullptr;
   double *herr_pin = nullptr;
+  // split-k workspaces of the skinny products (sized for cap columns)
+  DBuf<float> wk_b;
+  DBuf<double> wk_d;
   int nact = 0, nreq = 0, nbar = 0;
 
   std::vector<Slab *> slabs() {
     return {&acol, &F, &r0, &lu, &al, &ra, &Yc, &W, &Ad, &bcol, &G, &c0, &lv, &be, &sb,
-            &Zq, &X, &Bd, &kind, &base, &ns, &done, &mode, &maxit, &wcol, &tol, &errs,
-            &lists};
+            &Zq, &X, &Bd, &kind, &base, &ns, &done, &mode, &maxit, &wcol, &tol, &errs};
   }
 
   void layout() {
@@ -184,16 +211,57 @@ struct Group {
     Bd.per_col = 8 * (size_t)ldn;
     kind.per_col = base.per_col = ns.per_col = done.per_col = mode.per_col = maxit.per_col = 4;
     wcol.per_col = tol.per_col = errs.per_col = 8;
-    lists.per_col = 16;
   }
 
   size_t per_col_bytes() {
     size_t b = 0;
     for (Slab *s : slabs()) b += s->per_col;
-    return b + 2 * 16 + 12;  // pinned lists, flags and errors
+    return b + 16;  // device lists
+  }
+
+  // k splits of the four products: Z = K^T W (rows N, k M), Y = K X (rows M,
+  // k N) in BF16; r0 (rows M, k N), c0 (rows N, k M) in FP64
+  int ks_z() const { return splitk_for<uint16_t>(N, M); }
+  int ks_y() const { return splitk_for<uint16_t>(M, N); }
+  int ks_r() const { return splitk_for<double>(M, N); }
+  int ks_c() const { return splitk_for<double>(N, M); }
+  size_t wk_b_elems(int c) const {
+    return std::max(splitk_work_elems<uint16_t>(N, c, ks_z()),
+                    splitk_work_elems<uint16_t>(M, c, ks_y()));
+  }
+  size_t wk_d_elems(int c) const {
+    return std::max(splitk_work_elems<double>(M, c, ks_r()),
+                    splitk_work_elems<double>(N, c, ks_c()));
+  }
+
+  SplitK<uint16_t> skb(int ks) const {
+    SplitK<uint16_t> k;
+    k.ks = ks;
+    k.work = wk_b.get();
+    return k;
+  }
+  SplitK<double> skd(int ks) const {
+    SplitK<double> k;
+    k.ks = ks;
+    k.work = wk_d.get();
+    return k;
+  }
+
+  // Device bytes of the group at c columns (layout() set).
+  size_t bytes(int c) {
+    return per_col_bytes() * (size_t)c + 4 * wk_b_elems(c) + 8 * wk_d_elems(c);
+  }
+
+  void drop_graphs() {
+    for (hipGraphExec_t &g : gx)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
   }
 
   ~Group() {
+    drop_graphs();
     if (ev) (void)hipEventDestroy(ev);
     for (int *&h : hl)
       if (h) (void)hipHostFree(h);
@@ -206,15 +274,19 @@ struct Group {
   // first: the old buffers go back to the allocator cache.
   void grow(int ncap) {
     stream->sync();
+    drop_graphs();  // they hold the old buffers' addresses
     for (Slab *s : slabs()) {
       DBuf<char> nb(s->per_col * (size_t)ncap);
       if (cap) d2d(nb.get(), s->buf.get(), s->per_col * (size_t)cap, stream->get());
       stream->sync();
       s->buf = std::move(nb);
     }
+    lists.reset(4 + 4 * (size_t)ncap);
+    wk_b.reset(wk_b_elems(ncap));
+    wk_d.reset(wk_d_elems(ncap));
     for (int *&h : hl) {
       if (h) RWIG_HIP_CHECK(hipHostFree(h));
-      RWIG_HIP_CHECK(hipHostMalloc((void **)&h, sizeof(int) * 4 * (size_t)ncap));
+      RWIG_HIP_CHECK(hipHostMalloc((void **)&h, sizeof(int) * (4 + 4 * (size_t)ncap)));
     }
     if (hdone_pin) RWIG_HIP_CHECK(hipHostFree(hdone_pin));
     if (herr_pin) RWIG_HIP_CHECK(hipHostFree(herr_pin));
@@ -233,28 +305,70 @@ struct Group {
     return -1;
   }
 
-  // Uploads act | areq | riter | abar from the host lists (phase 0: before an
-  // absorption, 1: before iterations). Each phase has its own pinned buffer,
-  // which is rewritten only after a stream synchronisation that follows its
-  // previous upload.
-  void upload_lists(int phase, bool only_running) {
+  // Fills the pinned list buffer of a cycle phase (0: absorption, every
+  // running request; 1: iterations, the requests still running after the
+  // harvest). Each phase has its own buffer, rewritten only after a stream
+  // synchronisation that follows its previous upload.
+  void fill_lists(int phase) {
     int *h = hl[phase];
-    int *hact = h, *hreq = h + cap, *hit = h + 2 * cap, *hbar = h + 3 * cap;
+    int *hact = h + 4, *hreq = h + 4 + cap, *hit = h + 4 + 2 * cap, *hbar = h + 4 + 3 * cap;
     nact = nreq = nbar = 0;
     for (Request *r : running) {
-      if (only_running && r->state != kRunning) continue;
       hreq[nreq] = r->col;
       hit[nreq] = r->iter;
       ++nreq;
       if (r->kind == kBarycenter) hbar[nbar++] = r->col;
       for (int s = 0; s < r->S; ++s) hact[nact++] = r->col + s;
     }
-    h2d(lists.p<int>(), h, 4 * (size_t)cap, stream->get());
+    h[kNact] = nact;
+    h[kNreq] = nreq;
+    h[kNbar] = nbar;
+    h[3] = 0;
   }
-  const int *d_act() const { return lists.p<int>(); }
-  const int *d_areq() const { return lists.p<int>() + cap; }
-  const int *d_riter() const { return lists.p<int>() + 2 * cap; }
-  const int *d_abar() const { return lists.p<int>() + 3 * cap; }
+  void upload_lists(int phase) {
+    h2d(lists.get(), hl[phase], 4 + 4 * (size_t)cap, stream->get());
+  }
+  const int *d_cnt() const { return lists.get(); }
+  const int *d_act() const { return lists.get() + 4; }
+  const int *d_areq() const { return lists.get() + 4 + cap; }
+  const int *d_riter() const { return lists.get() + 4 + 2 * cap; }
+  const int *d_abar() const { return lists.get() + 4 + 3 * cap; }
+
+  // Runs one phase of a cycle on the stream: the list upload, then the
+  // absorption (phase 0) or absorb_every iterations (phase 1). With graphs,
+  // the phase is recorded once per column capacity and replayed: every
+  // launch reads its column counts and lists from device memory.
+  void run_phase(int phase) {
+    hipStream_t st = stream->get();
+    auto body = [&] {
+      upload_lists(phase);
+      if (phase == 0)
+        absorb();
+      else
+        for (int s = 0; s < absorb_every; ++s) iterate();
+    };
+    if (!graphs) {
+      body();
+    } else {
+      if (!gx[phase]) {
+        hipGraph_t g = nullptr;
+        RWIG_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        try {
+          body();
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        RWIG_HIP_CHECK(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&gx[phase], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        RWIG_HIP_CHECK(e);
+      }
+      RWIG_HIP_CHECK(hipGraphLaunch(gx[phase], st));
+    }
+    if (phase == 0) RWIG_HIP_CHECK(hipEventRecord(ev, st));
+  }
 
   // Fresh state of the columns [c0, c0 + S) of a newly admitted request.
   void init_request(const Request &r) {
@@ -309,13 +423,14 @@ struct Group {
   }
 
   // Fold the scalings into the potentials, exact FP64 marginals, stopping
-  // test, calibration of the BF16 products and reset of the scalings.
-  void absorb(double switch_thr) {
+  // test, calibration of the BF16 products and reset of the scalings; the
+  // stop flags and errors go to pinned host memory.
+  void absorb() {
     hipStream_t st = stream->get();
-    const int Mi = M, Ni = N, na = nact;
+    const int Mi = M, Ni = N, cp = cap;
     const long ldmi = ldm, ldni = ldn;
     const double rg = reg;
-    const int *act = d_act();
+    const int *act = d_act(), *cnt = d_cnt();
     double *Fp = F.p<double>(), *Gp = G.p<double>(), *Adp = Ad.p<double>(),
            *Bdp = Bd.p<double>(), *r0p = r0.p<double>(), *c0p = c0.p<double>(),
            *ap = acol.p<double>(), *bp = bcol.p<double>();
@@ -324,8 +439,8 @@ struct Group {
     uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
     int *dn = done.p<int>(), *md = mode.p<int>();
     const DenseKernel &dk = gk->dk;
-    launch_map(st, (long)Mi * na, [=] __device__(long k) {
-      const long t = act[k / Mi], i = k % Mi, q = i + t * Mi;
+    launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {
+      const long t = act[c], q = i + t * Mi;
       const double x = (double)lup[q];
       const double fk = Fp[q] + rg * (md[t] ? log1p(x) : log(x));
       Fp[q] = fk;
@@ -333,8 +448,8 @@ struct Group {
       alp[q] = (float)ak;
       Adp[i + t * ldmi] = ak;
     });
-    launch_map(st, (long)Ni * na, [=] __device__(long k) {
-      const long t = act[k / Ni], j = k % Ni, q = j + t * Ni;
+    launch_cols(st, Ni, cp, cnt, kNact, [=] __device__(int c, long j) {
+      const long t = act[c], q = j + t * Ni;
       const double x = (double)lvp[q];
       const double gq = Gp[q] + rg * (md[t] ? log1p(x) : log(x));
       Gp[q] = gq;
@@ -343,43 +458,43 @@ struct Group {
       Bdp[j + t * ldni] = bq;
     });
     gemm_tn<double>(
-        st, Mi, na, Ni, dk.KdT.get(), ldni, Bdp, ldni,
+        st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
         [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
-        act);
+        act, cnt + kNact, skd(ks_r()));
     gemm_tn<double>(
-        st, Ni, na, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
+        st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
         [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
-        act);
-    hipLaunchKernelGGL(k_serve_err, dim3(nreq), dim3(256), 0, st, Mi, Ni, rg, d_areq(),
-                       d_riter(), kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
+        act, cnt + kNact, skd(ks_c()));
+    hipLaunchKernelGGL(k_serve_err, dim3(cp), dim3(256), 0, st, Mi, Ni, rg, d_areq(), d_riter(),
+                       kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
                        tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,
-                       errs.p<double>());
+                       errs.p<double>(), cnt);
     check_launch();
     // calibration: the same BF16 products at u = v = 1
-    launch_map(st, (long)Mi * na, [=] __device__(long k) {
-      const long t = act[k / Mi], i = k % Mi;
+    launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {
+      const long t = act[c];
       Wp[i + t * ldmi] = f2bf(alp[i + t * Mi]);
     });
-    launch_map(st, (long)Ni * na, [=] __device__(long k) {
-      const long t = act[k / Ni], j = k % Ni;
+    launch_cols(st, Ni, cp, cnt, kNact, [=] __device__(int c, long j) {
+      const long t = act[c];
       Xp[j + t * ldni] = f2bf(bep[j + t * Ni]);
     });
     gemm_tn<uint16_t>(
-        st, Mi, na, Ni, dk.KbT.get(), ldni, Xp, ldni,
+        st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
         [=] __device__(int i, int t, float y) {
           if (!dn[t]) Ycp[i + (long)t * Mi] = y;
         },
-        act);
+        act, cnt + kNact, skb(ks_y()));
     gemm_tn<uint16_t>(
-        st, Ni, na, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+        st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
         [=] __device__(int j, int t, float z) {
           const long q = j + (long)t * Ni;
           if (!dn[t]) sbp[q] = (float)(c0p[q] / (double)z);
         },
-        act);
+        act, cnt + kNact, skb(ks_z()));
     // u = a / r0 at v = 1; W for the first product of the cycle
-    launch_map(st, (long)Mi * na, [=] __device__(long k) {
-      const long t = act[k / Mi], i = k % Mi, q = i + t * Mi;
+    launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {
+      const long t = act[c], q = i + t * Mi;
       if (dn[t]) return;
       rap[q] = (float)(r0p[q] / (double)Ycp[q]);
       const double u = ap[q] / r0p[q];
@@ -387,23 +502,22 @@ struct Group {
       lup[q] = md[t] ? (float)(u - 1.0) : (float)u;
       Wp[i + t * ldmi] = f2bf(md[t] ? (float)(a_ * (u - 1.0)) : (float)(a_ * u));
     });
-    launch_map(st, (long)Ni * na, [=] __device__(long k) {
-      const long t = act[k / Ni], j = k % Ni;
+    launch_cols(st, Ni, cp, cnt, kNact, [=] __device__(int c, long j) {
+      const long t = act[c];
       if (!dn[t]) lvp[j + t * Ni] = md[t] ? 0.0f : 1.0f;
     });
     d2h(hdone_pin, dn, (size_t)cap, st);
     d2h(herr_pin, errs.p<double>(), (size_t)cap, st);
-    RWIG_HIP_CHECK(hipEventRecord(ev, st));
   }
 
   // One iteration over the running columns: Z = K^T W -> v, X; for the
   // barycenters the common target and v; Y = K X -> u, W.
   void iterate() {
     hipStream_t st = stream->get();
-    const int Mi = M, Ni = N, na = nact, nb = nbar;
+    const int Mi = M, Ni = N, cp = cap;
     const long ldmi = ldm, ldni = ldn;
     const double rg = reg;
-    const int *act = d_act(), *abar = d_abar();
+    const int *act = d_act(), *abar = d_abar(), *cnt = d_cnt();
     const int *kd = kind.p<int>(), *md = mode.p<int>(), *np = ns.p<int>();
     const double *c0p = c0.p<double>(), *r0p = r0.p<double>(), *ap = acol.p<double>(),
                  *bp = bcol.p<double>(), *Gp = G.p<double>(), *wp = wcol.p<double>();
@@ -413,7 +527,7 @@ struct Group {
     uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
     const DenseKernel &dk = gk->dk;
     gemm_tn<uint16_t>(
-        st, Ni, na, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+        st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
         [=] __device__(int j, int t, float z) {
           const long q = j + (long)t * Ni;
           if (kd[t] == kBarycenter) {
@@ -432,31 +546,30 @@ struct Group {
             lvp[q] = (float)v;
           }
         },
-        act);
-    if (nb > 0)
-      launch_map(st, (long)Ni * nb, [=] __device__(long k) {
-        const long r = abar[k / Ni], j = k % Ni;
-        const int S = np[r], m1 = md[r];
-        double lb = 0.0;
-        for (int s = 0; s < S; ++s) {
-          const long q = j + (r + s) * Ni;
-          const double z = (double)Zqp[q];
-          const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
-          lb += wp[r + s] * (log(qs) - Gp[q] / rg);
-        }
-        const double bj = exp(lb);
-        for (int s = 0; s < S; ++s) {
-          const long q = j + (r + s) * Ni;
-          const double z = (double)Zqp[q];
-          const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
-          const double v = bj / qs;
-          const double b_ = (double)bep[q];
-          Xp[j + (r + s) * ldni] = f2bf(m1 ? (float)(b_ * (v - 1.0)) : (float)(b_ * v));
-          lvp[q] = m1 ? (float)(v - 1.0) : (float)v;
-        }
-      });
+        act, cnt + kNact, skb(ks_z()));
+    launch_cols(st, Ni, cp, cnt, kNbar, [=] __device__(int c, long j) {
+      const long r = abar[c];
+      const int S = np[r], m1 = md[r];
+      double lb = 0.0;
+      for (int s = 0; s < S; ++s) {
+        const long q = j + (r + s) * Ni;
+        const double z = (double)Zqp[q];
+        const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
+        lb += wp[r + s] * (log(qs) - Gp[q] / rg);
+      }
+      const double bj = exp(lb);
+      for (int s = 0; s < S; ++s) {
+        const long q = j + (r + s) * Ni;
+        const double z = (double)Zqp[q];
+        const double qs = m1 ? c0p[q] + (double)bep[q] * z : (double)sbp[q] * z;
+        const double v = bj / qs;
+        const double b_ = (double)bep[q];
+        Xp[j + (r + s) * ldni] = f2bf(m1 ? (float)(b_ * (v - 1.0)) : (float)(b_ * v));
+        lvp[q] = m1 ? (float)(v - 1.0) : (float)v;
+      }
+    });
     gemm_tn<uint16_t>(
-        st, Mi, na, Ni, dk.KbT.get(), ldni, Xp, ldni,
+        st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
         [=] __device__(int i, int t, float y) {
           const long q = i + (long)t * Mi;
           const double a_ = (double)alp[q];
@@ -471,40 +584,48 @@ struct Group {
             lup[q] = (float)u;
           }
         },
-        act);
+        act, cnt + kNact, skb(ks_y()));
   }
 
-  // Results of a finished request into its host vectors (stream synchronised).
-  void harvest(Request &r) {
+  // Copies of a finished request's results to the host (enqueued; complete
+  // after the next stream synchronisation).
+  void harvest_enqueue(Request &r) {
     hipStream_t st = stream->get();
-    const int Mi = M, Ni = N, c = r.col;
-    const double rg = reg;
+    const long c = r.col;
     r.err = herr_pin[c];
     if (r.kind == kSinkhorn) {
-      // regularised loss sum C.*P + reg sum P.*(log P - 1) of the absorbed
-      // plan (src/sinkhorn_impl_log.cpp:95-101)
-      const double *Cp = C, *Fp = F.p<double>() + (long)c * Mi, *Gp = G.p<double>() + (long)c * Ni;
-      big_sum(
-          st, (long)Mi * Ni,
-          [=] __device__(long k) {
-            const long i = k % Mi, j = k / Mi;
-            const double cij = Cp[k];
-            const double lp = (Fp[i] + Gp[j] - cij) / rg;
-            const double p = exp(lp);
-            return cij * p + rg * p * (lp - 1.0);
-          },
-          part.get(), scal.get());
-      r.out1.resize(Mi);
-      r.out2.resize(Ni);
-      d2h(r.out1.data(), Fp, (size_t)Mi, st);
-      d2h(r.out2.data(), Gp, (size_t)Ni, st);
-      d2h(&r.loss, scal.get(), 1, st);
+      r.out1.resize(M);
+      r.out2.resize(N);
+      r.r0.resize(M);
+      r.c0.resize(N);
+      d2h(r.out1.data(), F.p<double>() + c * M, (size_t)M, st);
+      d2h(r.out2.data(), G.p<double>() + c * N, (size_t)N, st);
+      d2h(r.r0.data(), r0.p<double>() + c * M, (size_t)M, st);
+      d2h(r.c0.data(), c0.p<double>() + c * N, (size_t)N, st);
     } else {
-      r.out1.resize(Ni);
-      d2h(r.out1.data(), bcol.p<double>() + (long)c * Ni, (size_t)Ni, st);
+      r.out1.resize(N);
+      d2h(r.out1.data(), bcol.p<double>() + c * N, (size_t)N, st);
     }
-    stream->sync();
-    if (r.kind == kBarycenter) {
+  }
+
+  // Host-side completion of a harvested request (after the synchronisation).
+  void harvest_finish(Request &r) {
+    if (r.kind == kSinkhorn) {
+      // regularised loss sum C.*P + reg sum P.*(log P - 1)
+      // (src/sinkhorn_impl_log.cpp:95-101) of the absorbed plan
+      // P = exp((f 1^T + 1 g^T - C) / reg): there C + reg (log P - 1) =
+      // f_i + g_j - reg, so the loss is f.r + g.c - reg sum(r) with r, c the
+      // plan's exact marginals from the last absorption
+      double fr = 0.0, gc = 0.0, tr = 0.0;
+      for (int i = 0; i < M; ++i) {
+        fr += r.out1[i] * r.r0[i];
+        tr += r.r0[i];
+      }
+      for (int j = 0; j < N; ++j) gc += r.out2[j] * r.c0[j];
+      r.loss = fr + gc - reg * tr;
+      r.r0.clear();
+      r.c0.clear();
+    } else {
       double t = 0.0;
       for (double x : r.out1) t += x;
       for (double &x : r.out1) x /= t;  // src/barycenter_impl_parallel.cpp:39-43
@@ -516,9 +637,9 @@ struct Group {
 
 class ServeEngine {
  public:
-  ServeEngine(double budget, int max_cols, int absorb_every, double switch_thr)
+  ServeEngine(double budget, int max_cols, int absorb_every, double switch_thr, bool graphs)
       : budget_(budget), max_cols_(max_cols), absorb_every_(absorb_every),
-        switch_thr_(switch_thr) {
+        switch_thr_(switch_thr), graphs_(graphs) {
     RWIG_HIP_CHECK(hipGetDevice(&device_));
     th_ = std::thread([this] { loop(); });
   }
@@ -670,7 +791,7 @@ class ServeEngine {
     probe.ldn = pad8(N);
     probe.layout();
     const int cols = std::max(kColPage, S);
-    return dense_kernel_bytes(M, N) + probe.per_col_bytes() * cols;
+    return dense_kernel_bytes(M, N) + probe.bytes(cols);
   }
 
   // Makes room for `need` bytes by releasing idle groups, then idle Gibbs
@@ -710,7 +831,7 @@ class ServeEngine {
 
   void release_group(Group *g) {
     g->stream->sync();
-    charge(-(long)(g->per_col_bytes() * (size_t)g->cap));
+    charge(-(long)g->bytes(g->cap));
     g->gk->users -= 1;
     groups_.erase(std::find_if(groups_.begin(), groups_.end(),
                                [&](const std::unique_ptr<Group> &p) { return p.get() == g; }));
@@ -742,12 +863,11 @@ class ServeEngine {
     probe.ldm = pad8(c.M);
     probe.ldn = pad8(c.N);
     probe.layout();
-    const size_t pc = probe.per_col_bytes();
     if (g) {
       int ncap = g->cap;
       while (ncap < max_cols_ && g->find_cols(r->S) < 0) {
         ncap = std::min(max_cols_, std::max(ncap * 2, ncap + r->S));
-        const size_t need = pc * (size_t)(ncap - g->cap);
+        const size_t need = g->bytes(ncap) - g->bytes(g->cap);
         if (!make_room(need, g->gk)) return false;
         charge((long)need);
         g->grow(ncap);
@@ -758,20 +878,20 @@ class ServeEngine {
     }
     const int cap0 = std::min(max_cols_, std::max(kColPage, (r->S + kColPage - 1) / kColPage * kColPage));
     const size_t kbytes = k ? 0 : dense_kernel_bytes(c.M, c.N);
-    if (!make_room(kbytes + pc * (size_t)cap0, k)) return false;
+    if (!make_room(kbytes + probe.bytes(cap0), k)) return false;
     auto ng = std::make_unique<Group>();
     ng->cost = r->cost;
     ng->reg = r->reg;
-    ng->C = c.C.get();
     ng->M = c.M;
     ng->N = c.N;
     ng->ldm = pad8(c.M);
     ng->ldn = pad8(c.N);
     ng->layout();
+    ng->graphs = graphs_;
+    ng->absorb_every = absorb_every_;
+    ng->switch_thr = switch_thr_;
     ng->stream = std::make_unique<Stream>();
     RWIG_HIP_CHECK(hipEventCreateWithFlags(&ng->ev, hipEventDisableTiming));
-    ng->part.reset(kSumBlocks);
-    ng->scal.reset(1);
     if (!k) {
       auto nk = std::make_unique<Gibbs>();
       nk->cost = r->cost;
@@ -784,7 +904,7 @@ class ServeEngine {
     }
     k->users += 1;
     ng->gk = k;
-    charge((long)(pc * (size_t)cap0));
+    charge((long)ng->bytes(cap0));
     ng->grow(cap0);
     g = ng.get();
     groups_.push_back(std::move(ng));
@@ -831,30 +951,37 @@ class ServeEngine {
     for (Group *g : active) {
       g->last = cycles_;
       g->gk->last = cycles_;
-      g->upload_lists(0, false);
-      g->absorb(switch_thr_);
+      g->fill_lists(0);
+      g->run_phase(0);
     }
     for (Group *g : active) {
       RWIG_HIP_CHECK(hipEventSynchronize(g->ev));
-      std::vector<Request *> keep;
+      std::vector<Request *> keep, fin;
       for (Request *r : g->running) {
         if (g->hdone_pin[r->col]) {
-          g->harvest(*r);
+          g->harvest_enqueue(*r);
+          fin.push_back(r);
+        } else {
+          keep.push_back(r);
+        }
+      }
+      if (!fin.empty()) {
+        g->stream->sync();
+        for (Request *r : fin) {
+          g->harvest_finish(*r);
           for (int s = 0; s < r->S; ++s) g->owner[r->col + s] = nullptr;
           {
             std::lock_guard<std::mutex> lk(mu_);
             --running_;
           }
           finish(r, true, "");
-          cv_done_.notify_all();
-        } else {
-          keep.push_back(r);
         }
+        cv_done_.notify_all();
       }
       g->running.swap(keep);
       if (g->running.empty()) continue;
-      g->upload_lists(1, true);
-      for (int s = 0; s < absorb_every_; ++s) g->iterate();
+      g->fill_lists(1);
+      g->run_phase(1);
       for (Request *r : g->running) r->iter += absorb_every_;
     }
   }
@@ -919,6 +1046,7 @@ class ServeEngine {
   double budget_;
   int max_cols_, absorb_every_;
   double switch_thr_;
+  bool graphs_;
   int device_ = 0;
   // shared state (mu_)
   std::mutex mu_;
@@ -958,7 +1086,7 @@ ServeEngine *engine_of(void *h) {
 extern "C" {
 
 int rwig_serve_create(double hbm_budget_bytes, int max_group_columns, int absorb_every,
-                      double switch_thr, void **session) {
+                      double switch_thr, int flags, void **session) {
   return guarded("rwig_serve_create", [&] {
     RWIG_REQUIRE(session != nullptr, "null session pointer");
     RWIG_REQUIRE(hbm_budget_bytes > 0, "HBM budget must be positive");
@@ -966,7 +1094,23 @@ int rwig_serve_create(double hbm_budget_bytes, int max_group_columns, int absorb
                  "max_group_columns must be in [1, 4096]");
     RWIG_REQUIRE(absorb_every >= 1, "absorb_every must be positive");
     ensure_device();
-    *session = new ServeEngine(hbm_budget_bytes, max_group_columns, absorb_every, switch_thr);
+    RWIG_REQUIRE((flags & ~RWIG_SERVE_EAGER) == 0, "unknown flags");
+    *session = new ServeEngine(hbm_budget_bytes, max_group_columns, absorb_every, switch_thr,
+                               (flags & RWIG_SERVE_EAGER) == 0);
+  });
+}
+
+int rwig_serve_footprint(int M, int N, int columns, double *kernel_bytes, double *group_bytes) {
+  return guarded("rwig_serve_footprint", [&] {
+    RWIG_REQUIRE(M > 0 && N > 0 && columns > 0, "invalid argument");
+    Group probe;
+    probe.M = M;
+    probe.N = N;
+    probe.ldm = pad8(M);
+    probe.ldn = pad8(N);
+    probe.layout();
+    if (kernel_bytes) *kernel_bytes = (double)dense_kernel_bytes(M, N);
+    if (group_bytes) *group_bytes = (double)probe.bytes(columns);
   });
 }
 

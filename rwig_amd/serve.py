@@ -28,14 +28,24 @@ def _f64(x):
     return np.asfortranarray(np.asarray(x, dtype=np.float64))
 
 
+def footprint(M, N, columns=32):
+    """(Gibbs kernel bytes, group bytes at `columns` columns) that a session
+    charges against its HBM budget for one (cost, reg) pair."""
+    kb, gb = ctypes.c_double(), ctypes.c_double()
+    check(lib().rwig_serve_footprint(int(M), int(N), int(columns), ctypes.byref(kb),
+                                     ctypes.byref(gb)))
+    return kb.value, gb.value
+
+
 class ServeSession:
     """rwig_serve_create / _destroy; use as a context manager."""
 
     def __init__(self, hbm_budget_bytes=8 << 30, max_group_columns=512, absorb_every=20,
-                 switch_thr=1e-2):
+                 switch_thr=1e-2, graphs=True):
         h = ctypes.c_void_p()
         check(lib().rwig_serve_create(float(hbm_budget_bytes), int(max_group_columns),
-                                      int(absorb_every), float(switch_thr), ctypes.byref(h)))
+                                      int(absorb_every), float(switch_thr),
+                                      0 if graphs else 1, ctypes.byref(h)))
         self._h = h
         self._shapes = {}
         self._costs = {}

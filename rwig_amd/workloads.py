@@ -71,3 +71,38 @@ def c3(grid=64, S=8, seed=1003):
     np.fill_diagonal(C, 0.0)
     C /= C.max()
     return np.asfortranarray(A), np.asfortranarray(C), w, b_ext, 0.004, 200
+
+
+def c5_clouds(n_clouds=32, seed=1005):
+    """Serving mix, cost side: n_clouds point clouds of n_k points uniform in
+    [0,1]^2, n_k = round(exp(U[ln 64, ln 8192])), squared Euclidean cost.
+    Returns the list of cost matrices (n_k x n_k, Fortran order)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(64), np.log(8192), n_clouds))).astype(int)
+    out = []
+    for n in sizes:
+        X = rng.random((int(n), 2))
+        sq = (X * X).sum(1)
+        C = np.maximum(sq[:, None] + sq[None, :] - 2.0 * X @ X.T, 0.0)
+        np.fill_diagonal(C, 0.0)
+        out.append(np.asfortranarray(C))
+    return out
+
+
+def c5_requests(sizes, count, seed=2005):
+    """Serving mix, request side: 70% Sinkhorn (a, b ~ Dirichlet(1) on a
+    random cloud), 30% barycenter (S in {2, 4, 8}, columns of A and the
+    weights ~ Dirichlet(1)), eps in {0.1, 0.01}. Yields dicts."""
+    rng = np.random.default_rng(seed)
+    for _ in range(count):
+        k = int(rng.integers(len(sizes)))
+        n = int(sizes[k])
+        eps = float(rng.choice([0.1, 0.01]))
+        if rng.random() < 0.7:
+            yield {"kind": "sinkhorn", "cloud": k, "eps": eps,
+                   "a": rng.dirichlet(np.ones(n)), "b": rng.dirichlet(np.ones(n))}
+        else:
+            S = int(rng.choice([2, 4, 8]))
+            yield {"kind": "barycenter", "cloud": k, "eps": eps,
+                   "A": np.asfortranarray(rng.dirichlet(np.ones(n), size=S).T),
+                   "w": rng.dirichlet(np.ones(S))}

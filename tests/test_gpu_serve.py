@@ -17,7 +17,7 @@ import pytest
 
 from oracle import oracle as O
 from rwig_amd import RwigError
-from rwig_amd.serve import DONE, ServeSession
+from rwig_amd.serve import DONE, ServeSession, footprint
 
 pytestmark = pytest.mark.gpu
 
@@ -135,7 +135,10 @@ def test_admission_by_hbm_budget(clouds):
     # requests queue until an idle group is released, within the budget
     rng = np.random.default_rng(4)
     cost_bytes = sum(C.nbytes for C in clouds)
-    budget = cost_bytes + int(12.5 * 2**20)  # the 700-point group fits, not with the others
+    k1, g1 = footprint(700, 700, 32)
+    k0, g0 = footprint(200, 200, 32)
+    # the 700-point group fits, not together with the 200-point one
+    budget = int(cost_bytes + k1 + g1 + 0.5 * (k0 + g0))
     with ServeSession(hbm_budget_bytes=budget, max_group_columns=64) as s:
         ids = [s.register_cost(C) for C in clouds]
         jobs = []
@@ -180,3 +183,37 @@ def test_max_iter_and_invalid_input(clouds):
             s.submit_sinkhorn(cid, bad, b, 0.1)
         with pytest.raises(RwigError):
             s.submit_sinkhorn(cid + 5, a, b, 0.1)
+
+
+def test_graph_replay_equals_eager(clouds):
+    # the cycle phases replayed as HIP graphs (default) give the bits of the
+    # same launches made directly, over many replays with changing column sets
+    rng = np.random.default_rng(6)
+    reqs = []
+    for k in range(10):
+        c = k % 2
+        n = clouds[c].shape[0]
+        eps = 0.01 if k % 3 == 0 else 0.1
+        if k % 4 == 1:
+            S = 2 + 2 * (k % 2)
+            reqs.append(("b", c, eps, rng.dirichlet(np.ones(n), size=S).T.copy(),
+                         rng.dirichlet(np.ones(S))))
+        else:
+            reqs.append(("s", c, eps, rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))))
+    outs = []
+    for graphs in (False, True):
+        with ServeSession(hbm_budget_bytes=2 << 30, graphs=graphs) as s:
+            ids = [s.register_cost(C) for C in clouds]
+            rids = []
+            for kind, c, eps, x, y in reqs:
+                if kind == "s":
+                    rids.append(s.submit_sinkhorn(ids[c], x, y, eps))
+                else:
+                    rids.append(s.submit_barycenter(ids[c], x, y, eps))
+            outs.append([s.result(r) for r in rids])
+            assert s.stats()["cycles"] > 10
+    for e, g in zip(*outs):
+        assert e["iter"] == g["iter"] and e["err"] == g["err"]
+        for key in ("f", "g", "b"):
+            if key in e:
+                assert np.array_equal(e[key], g[key]), key

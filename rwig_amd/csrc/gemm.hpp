@@ -283,61 +283,58 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
     cur ^= 1;
   }
   if (sk.work) {
-    // partial tile of this split; the sum and the epilogue follow in
-    // k_gemm_splitk_epilogue (the kernel boundary orders the writes)
-    constexpr int NE = TM * TN * 4;
+    // partial tile of this split, column-major within the tile; the sum and
+    // the epilogue follow in k_gemm_splitk_epilogue (the kernel boundary
+    // orders the writes)
     const long ntile = (long)gridDim.x * gridDim.y;
     const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
-    typename MF::out_t *w = sk.work + ((long)blockIdx.z * ntile + tile) * NE * 256 + threadIdx.x;
+    typename MF::out_t *w = sk.work + ((long)blockIdx.z * ntile + tile) * BM * BN;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w[((i * TN + j) * 4 + r) * 256] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r) {
+          const int nl = wn * WN + j * 16 + lr, ml = wm * WM + i * 16 + MF::row(lane, r);
+          w[nl * BM + ml] = acc[i][j][r];
+        }
     return;
   }
   gemm_epilogue<T>(M, N, m0, n0, nmap, acc, op);
 }
 
 // Sum of the split partials of each tile, in split order, then the epilogue.
-// Same tile and thread layout as the product, with one 16 x 16 fragment
-// (i, j) = blockIdx.z per workgroup: TM x TN times the product's workgroups,
-// so that the epilogue's loads have the occupancy to hide their latency.
+// Workgroup (tile, blockIdx.z) covers 16 columns of the tile with its
+// threads along the rows, so that the epilogue's column-major loads and
+// stores are contiguous across a wavefront.
+constexpr int kEpiCols = 16;
+
 template <typename T, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
                        const int *__restrict__ ncount, int ks,
                        const typename Mfma<T>::out_t *__restrict__ work, Op op) {
-  using MF = Mfma<T>;
   constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
-  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16, NE = TM * TN * 4;
   if (ncount) {
     const int nc = *ncount;
     if ((int)blockIdx.y * BN >= nc) return;
     N = nc;
   }
-  const int i = blockIdx.z / TN, j = blockIdx.z % TN;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wm = wv & 1, wn = wv >> 1;
-  const int n = blockIdx.y * BN + wn * WN + j * 16 + (lane & 15);
-  if (n >= N) return;
   const long ntile = (long)gridDim.x * gridDim.y;
   const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
-  const long zs = ntile * NE * 256;
-  const typename MF::out_t *w0 = work + tile * NE * 256 + ((i * TN + j) * 4) * 256 + threadIdx.x;
-  typename MF::out_t v[4];
+  const long zs = ntile * BM * BN;
+  const typename Mfma<T>::out_t *w0 = work + tile * BM * BN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    typename MF::out_t x = w0[r * 256];
-    for (int z = 1; z < ks; ++z) x += w0[z * zs + r * 256];
-    v[r] = x;
-  }
-  const int nn = nmap ? nmap[n] : n;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int m = blockIdx.x * BM + wm * WM + i * 16 + MF::row(lane, r);
-    if (m < M) op(m, nn, v[r]);
+  for (int pass = 0; pass < BM * kEpiCols / 256; ++pass) {
+    const int idx = pass * 256 + threadIdx.x;
+    const int nl = blockIdx.z * kEpiCols + idx / BM, ml = idx % BM;
+    const int m = m0 + ml, n = n0 + nl;
+    if (m >= M || n >= N) continue;
+    const long e = (long)nl * BM + ml;
+    typename Mfma<T>::out_t v = w0[e];
+    for (int z = 1; z < ks; ++z) v += w0[z * zs + e];
+    op(m, nmap ? nmap[n] : n, v);
   }
 }
 
@@ -362,9 +359,8 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
                      ncount, sk, op);
   check_launch();
   if (sk.work) {
-    constexpr int NF = (GemmCfg<T>::BM / 32) * (GemmCfg<T>::BN / 32);
-    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, Op>), dim3(grid.x, grid.y, NF), dim3(256), 0,
-                       st, M, N, nmap, ncount, sk.ks, sk.work, op);
+    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, Op>), dim3(grid.x, grid.y, BN / kEpiCols),
+                       dim3(256), 0, st, M, N, nmap, ncount, sk.ks, sk.work, op);
     check_launch();
   }
 }

@@ -35,10 +35,195 @@ METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TF = 2500.0
 ABSORB_MFMA = 30  # iterations between FP64 absorptions on the MFMA path
+GEMM_KSPLIT = 2   # k split of the MFMA path's BF16 products
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the dominant product from the committed counter
+    runs (profiles/c2_gemm_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_gemm_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)["traffic_bytes"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def c2_gemm(torch, dev, stream):
+    """The dominant kernel of the MFMA path at its full shape: K^T W with
+    M = K = 4096 (the cost's points) and N = 1024 (the targets), BF16 inputs,
+    k split in two, deferred epilogue. Returns (launch, flops per launch)."""
+    from rwig_amd._lib import check, lib
+    gM, gN, gK = 4096, 1024, 4096
+    Wt = torch.randn((gN, gK), device=dev).to(torch.bfloat16)
+    Kt = torch.randn((gM, gK), device=dev).to(torch.bfloat16)
+    Dt = torch.empty((gN, gM), dtype=torch.float32, device=dev)
+    nb = ctypes.c_long()
+    check(lib().rwig_gemm_workspace_bytes(0, gM, gN, GEMM_KSPLIT, ctypes.byref(nb)))
+    wk = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+
+    def launch():
+        check(lib().rwig_gemm_tn_bf16_dev(gM, gN, gK, Kt.data_ptr(), gK, Wt.data_ptr(), gK,
+                                          Dt.data_ptr(), gM, GEMM_KSPLIT, wk.data_ptr(),
+                                          stream.cuda_stream))
+    return launch, 2.0 * gM * gN * gK
+
+
+def gemm_only():
+    import torch
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    launch, _ = c2_gemm(torch, dev, stream)
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        launch()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print("gemm-only: 5 products 4096 x 1024 x 4096 (k split 2), 5 copies of 256 MiB",
+          flush=True)
+
+
+def c3_leg(log):
+    """Barycenter with unrolled gradient (64 x 64 grid, S = 8, eps 0.004,
+    L = 200, checkpoints every 20): one forward + adjoint in FP32 and FP64."""
+    import rwig_amd
+    from rwig_amd.workloads import c3
+    A, C, w, bext, eps, L = c3()
+    out = {"workload": "log barycenter + checkpointed adjoint, 64x64 grid (n=4096), S=8, "
+                       "eps=0.004, L=200, checkpoint every 20, seed unnormalised"}
+    for prec in ("fp32", "fp64"):
+        kw = dict(reg=eps, max_iter=L, zero_tol=0.0, with_grad=True, seed="unnormalised",
+                  checkpoint_every=20, precision=prec)
+        rwig_amd.barycenter_log_ck(A, C, w, bext, **kw)  # warm-up
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            rwig_amd.barycenter_log_ck(A, C, w, bext, **kw)
+        out[f"{prec}_ms_per_gradient"] = (time.perf_counter() - t) / reps * 1e3
+        log(f"C3 {prec}: {out[f'{prec}_ms_per_gradient']:.1f} ms per forward + adjoint "
+            "(host FP64 inputs and outputs included)")
+    return out
+
+
+def c4_leg(torch, dev, stream, log):
+    """WDL training step (V = 2000, K = 16 atoms, 512 documents, eps 0.1, L = 20):
+    one batched gradient over all documents (the step's dominant work; the Adam
+    update is O(V K + K D)), FP32 and FP64, inputs resident in HBM."""
+    from rwig_amd._lib import check, lib
+    from rwig_amd.workloads import c4
+    C, Y, Alpha0, Lambda0, eps, L = c4()
+    V, D = Y.shape
+    S = Alpha0.shape[1]
+
+    def softmax_cols(X):
+        e = np.exp(X - X.max(0, keepdims=True))
+        return e / e.sum(0, keepdims=True)
+
+    K = np.exp(-C / eps)
+    A = softmax_cols(Alpha0)
+    Wb = softmax_cols(Lambda0)
+    ldk = (V + 7) // 8 * 8
+    out = {"workload": f"WDL batched gradient, V={V}, K={S} atoms, {D} documents, eps={eps}, "
+                       f"L={L}"}
+    for prec, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        Kt = torch.zeros((V, ldk), dtype=dt)
+        Kt[:, :V] = torch.from_numpy(K.T.copy())
+        KTt = torch.zeros((V, ldk), dtype=dt)
+        KTt[:, :V] = torch.from_numpy(K.copy())
+        Kd, KTd = Kt.to(dev), KTt.to(dev)
+        At = torch.from_numpy(A.T.copy()).to(dt).to(dev)
+        Yt = torch.from_numpy(Y.T.copy()).to(dt).to(dev)
+        Wt = torch.from_numpy(Wb.T.copy()).to(dt).to(dev)
+        gA = torch.zeros((S, V), dtype=torch.float64, device=dev)
+        gl = torch.zeros(S, dtype=torch.float64, device=dev)
+        loss = torch.zeros(1, dtype=torch.float64, device=dev)
+        p = 0 if prec == "fp64" else 1
+
+        def step():
+            check(lib().rwig_wdl_batch_grad_dev(Kd.data_ptr(), KTd.data_ptr(), ldk, V, S, D, L,
+                                                At.data_ptr(), Yt.data_ptr(), Wt.data_ptr(),
+                                                gA.data_ptr(), gl.data_ptr(), loss.data_ptr(), p,
+                                                stream.cuda_stream))
+        step()
+        torch.cuda.synchronize()
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / reps * 1e3
+        out[f"{prec}_ms_per_step"] = ms
+        out[f"{prec}_docs_per_s"] = D / ms * 1e3
+        log(f"C4 {prec}: {ms:.2f} ms per batched gradient ({D / ms * 1e3:.0f} documents/s)")
+    return out
+
+
+def c5_leg(log, paced_s=8.0):
+    """Serving mix: 32 cost matrices (n log-uniform in [64, 8192]) resident,
+    70% Sinkhorn / 30% barycenter (S in {2, 4, 8}), eps in {0.1, 0.01}, stop at
+    L1 marginal error 1e-6. Saturated throughput (1024 requests at once), then
+    Poisson arrivals at 70% of it for paced_s seconds: p50/p99 latency."""
+    from rwig_amd import workloads as W
+    from rwig_amd.serve import ServeSession
+    clouds = W.c5_clouds()
+    sizes = [C.shape[0] for C in clouds]
+
+    def submit(s, ids, q):
+        if q["kind"] == "sinkhorn":
+            return s.submit_sinkhorn(ids[q["cloud"]], q["a"], q["b"], q["eps"])
+        return s.submit_barycenter(ids[q["cloud"]], q["A"], q["w"], q["eps"])
+
+    out = {"workload": "serving mix: 32 cost matrices n in [64, 8192] (log-uniform), 70% "
+                       "Sinkhorn / 30% barycenter S in {2,4,8}, eps in {0.1, 0.01}, "
+                       "L1 marginal err <= 1e-6, BF16 MFMA + FP64 absorption every 20"}
+    with ServeSession(hbm_budget_bytes=64 << 30, max_group_columns=512, absorb_every=20) as s:
+        ids = [s.register_cost(C) for C in clouds]
+        warm = [s.submit_sinkhorn(ids[k], np.full(n, 1.0 / n), np.full(n, 1.0 / n), eps)
+                for k, n in enumerate(sizes) for eps in (0.1, 0.01)]
+        for r in warm:
+            s.result(r)
+        reqs = list(W.c5_requests(sizes, 1024, seed=2005))
+        s.reset_stats()
+        t = time.perf_counter()
+        rids = [submit(s, ids, q) for q in reqs]
+        res = [s.result(r) for r in rids]
+        sat = len(reqs) / (time.perf_counter() - t)
+        ok = all(r["err"] <= 1e-6 for r in res)
+        st = s.stats()
+        out.update(saturated_solves_per_s=sat, saturated_p50_ms=st["p50_ms"],
+                   saturated_p99_ms=st["p99_ms"], all_converged_saturated=ok)
+        log(f"C5 saturated: {sat:.0f} solves/s, p50 {st['p50_ms']:.0f} ms, "
+            f"p99 {st['p99_ms']:.0f} ms, all converged {ok}")
+        rate = 0.7 * sat
+        rng = np.random.default_rng(2006)
+        reqs = list(W.c5_requests(sizes, int(rate * paced_s), seed=2007))
+        s.reset_stats()
+        t0 = time.perf_counter()
+        tnext = t0
+        rids = []
+        for q in reqs:
+            tnext += rng.exponential(1.0 / rate)
+            d = tnext - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            rids.append(submit(s, ids, q))
+        res = [s.result(r) for r in rids]
+        st = s.stats()
+        ok = all(r["err"] <= 1e-6 for r in res)
+        out.update(arrival_rate_per_s=rate, requests=len(reqs),
+                   paced_solves_per_s=st["solves_per_s"], p50_ms=st["p50_ms"],
+                   p99_ms=st["p99_ms"], mean_queue_ms=st["mean_queue_ms"],
+                   peak_hbm_bytes=st["peak_bytes"], all_converged=ok)
+        log(f"C5 paced at {rate:.0f}/s: {st['solves_per_s']:.0f} solves/s, "
+            f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, all converged {ok}")
+    return out
 
 
 def main():
@@ -50,11 +235,19 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the barycenter-gradient, WDL and serving measurements")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the dominant product (5 launches) and a 256 MiB copy (5 "
+                         "launches): the workload of the HBM-traffic counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched "
               "solves shard by target with no exchange, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.gemm_only:
+        gemm_only()
+        return
 
     import torch
 
@@ -115,14 +308,7 @@ def main():
 
     # dominant kernel: the BF16 MFMA GEMM K^T W (M x B x M); its average launch
     # time measured here with HIP events on the same stream, same shape
-    gM, gN, gK = M, B, M
-    Wt = torch.randn((gN, (gK + 7) // 8 * 8), device=dev).to(torch.bfloat16)
-    Kt = torch.randn((gM, (gK + 7) // 8 * 8), device=dev).to(torch.bfloat16)
-    Dt = torch.empty((gN, gM), dtype=torch.float32, device=dev)
-
-    def gemm():
-        check(lib().rwig_gemm_tn_bf16_dev(gM, gN, gK, Kt.data_ptr(), Kt.shape[1], Wt.data_ptr(),
-                                          Wt.shape[1], Dt.data_ptr(), gM, stream.cuda_stream))
+    gemm, gemm_flops = c2_gemm(torch, dev, stream)
 
     gemm()
     torch.cuda.synchronize()
@@ -134,7 +320,6 @@ def main():
     g1.record(stream)
     torch.cuda.synchronize()
     gemm_ms = g0.elapsed_time(g1) / reps
-    gemm_flops = 2.0 * gM * gN * gK
     achieved_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
 
     # ---------------------------------------------------------------- other forms
@@ -199,6 +384,12 @@ def main():
                          f"(C++ oracle, n_threads={cores}, zero_tol=1e-6 on the "
                          "reference error), n=4096 x 205 reduced problems"}
 
+    extra = {}
+    if not args.no_extra:
+        extra["c3_barycenter_gradient"] = c3_leg(log)
+        extra["c4_wdl_step"] = c4_leg(torch, dev, stream, log)
+        extra["c5_serving"] = c5_leg(log)
+
     res = {
         "metric": METRIC,
         "value": value,
@@ -236,9 +427,16 @@ def main():
         "speedup_vs_fp64_batched": value / fp64_sps,
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": MFMA_BF16_PEAK_TF,
                      "unit": "TFLOP/s", "frac": achieved_tf / MFMA_BF16_PEAK_TF,
-                     "traffic": None, "kernel": "k_gemm_tn<bf16> (K^T W, 4096 x 1024 x 4096)",
+                     "traffic": pmc_traffic(),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
+                                       "product, copy-calibrated "
+                                       "(profiles/c2_gemm_hbm_traffic.md)",
+                     "alg_bytes_per_launch": (4096 * 4096 + 1024 * 4096) * 2 + 4096 * 1024 * 4,
+                     "kernel": "k_gemm_tn<bf16> + k_gemm_splitk_epilogue (K^T W, 4096 x 1024 x "
+                               "4096, k split 2, as in the MFMA path)",
                      "kernel_ms": gemm_ms, "alg_flops_per_launch": gemm_flops},
         "cpu_baseline": cpu,
+        **extra,
     }
     print(json.dumps(res), flush=True)
 

@@ -223,12 +223,17 @@ int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc,
  * counterpart in the reference, which calls cuBLAS dgemm,
  * src/cuda_kernels.cu:563). bf16: A, B are BF16 bit patterns, D is FP32
  * (v_mfma_f32_16x16x32_bf16); f64: all FP64 (v_mfma_f64_16x16x4_f64). */
+/* ks >= 1 splits k over ks workgroups per tile; with a workspace (work,
+ * rwig_gemm_workspace_bytes(dtype, M, N, ks) bytes, dtype 0 bf16, 1 f64)
+ * the partial tiles are summed in split order by a second kernel, which also
+ * writes D; without one (work null, ks 1) the product kernel writes D. */
+int rwig_gemm_workspace_bytes(int dtype, int M, int N, int ks, long *bytes);
 int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda,
-                          const void *B, long ldb, float *D, long ldd,
-                          void *stream);
+                          const void *B, long ldb, float *D, long ldd, int ks,
+                          void *work, void *stream);
 int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
                          const double *B, long ldb, double *D, long ldd,
-                         void *stream);
+                         int ks, void *work, void *stream);
 
 /* ---------------------------------------------------------------- serving
  * A serving session: continuous batching of Sinkhorn and barycenter solves

@@ -38,10 +38,11 @@ SIGNATURES = {
                                    _vp], _i),
     "rwig_sinkhorn_log_dev": ([_vp, _vp, _vp, _i, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp,
                                _ip, _dp, _vp], _i),
+    "rwig_gemm_workspace_bytes": ([_i, _i, _i, _i, ctypes.POINTER(ctypes.c_long)], _i),
     "rwig_gemm_tn_bf16_dev": ([_i, _i, _i, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp,
-                               ctypes.c_long, _vp], _i),
+                               ctypes.c_long, _i, _vp, _vp], _i),
     "rwig_gemm_tn_f64_dev": ([_i, _i, _i, _vp, ctypes.c_long, _vp, ctypes.c_long, _vp,
-                              ctypes.c_long, _vp], _i),
+                              ctypes.c_long, _i, _vp, _vp], _i),
     "rwig_dense_kernel_create": ([_vp, ctypes.c_long, _i, _i, _d, _vp,
                                   ctypes.POINTER(ctypes.c_void_p)], _i),
     "rwig_dense_kernel_destroy": ([_vp], _i),

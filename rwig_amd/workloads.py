@@ -106,3 +106,31 @@ def c5_requests(sizes, count, seed=2005):
             yield {"kind": "barycenter", "cloud": k, "eps": eps,
                    "A": np.asfortranarray(rng.dirichlet(np.ones(n), size=S).T),
                    "w": rng.dirichlet(np.ones(S))}
+
+
+def c4(V=2000, dim=50, K=16, D=512, seed=1004):
+    """WDL training step: V words with dim-d embeddings ~ N(0, I), rows L2
+    normalised, squared Euclidean cost; K atoms; D documents with lengths
+    max(1, Poisson(60)) and word ids from a Zipf(1.1) law truncated to
+    1..V, as normalised bags of words; eps 0.1, L = 20.
+
+    Returns (C (V x V), Y (V x D), Alpha0 (V x K), Lambda0 (K x D), eps, L),
+    Fortran order."""
+    rng = np.random.default_rng(seed)
+    E = rng.standard_normal((V, dim))
+    E /= np.linalg.norm(E, axis=1, keepdims=True)
+    sq = (E * E).sum(1)
+    C = np.maximum(sq[:, None] + sq[None, :] - 2.0 * E @ E.T, 0.0)
+    np.fill_diagonal(C, 0.0)
+    p = np.arange(1, V + 1, dtype=float) ** -1.1
+    p /= p.sum()
+    Y = np.zeros((V, D))
+    for d in range(D):
+        n = max(1, int(rng.poisson(60)))
+        ids = rng.choice(V, size=n, p=p)
+        np.add.at(Y[:, d], ids, 1.0)
+    Y /= Y.sum(0, keepdims=True)
+    Alpha0 = rng.standard_normal((V, K))
+    Lambda0 = rng.standard_normal((K, D))
+    return (np.asfortranarray(C), np.asfortranarray(Y), np.asfortranarray(Alpha0),
+            np.asfortranarray(Lambda0), 0.1, 20)

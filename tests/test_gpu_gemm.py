@@ -15,17 +15,31 @@ torch = pytest.importorskip("torch")
 from rwig_amd._lib import check, lib  # noqa: E402
 
 
-def run_bf16(A, Bt, M, N, K):
+def workspace(dtype, M, N, ks):
+    import ctypes
+    nb = ctypes.c_long()
+    check(lib().rwig_gemm_workspace_bytes(dtype, M, N, ks, ctypes.byref(nb)))
+    return torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+
+
+def run_bf16(A, Bt, M, N, K, ks=1, deferred=False):
     # A: (M, lda) bf16 row-major; Bt: (N, ldb) bf16 (column n of B is row n)
     D = torch.empty((N, M), dtype=torch.float32, device="cuda")
+    w = workspace(0, M, N, ks) if deferred else None
     check(lib().rwig_gemm_tn_bf16_dev(M, N, K, A.data_ptr(), A.shape[1], Bt.data_ptr(),
-                                      Bt.shape[1], D.data_ptr(), M,
+                                      Bt.shape[1], D.data_ptr(), M, ks,
+                                      None if w is None else w.data_ptr(),
                                       torch.cuda.current_stream().cuda_stream))
     return D
 
 
+# (ks, deferred): fused epilogue; deferred epilogue; k split in 2 and 5
+SPLITS = [(1, False), (1, True), (2, True), (5, True)]
+
+
+@pytest.mark.parametrize("ks,deferred", SPLITS)
 @pytest.mark.parametrize("M,N,K", [(16, 16, 32), (128, 128, 64), (300, 70, 200), (1000, 257, 999)])
-def test_bf16_gemm_matches_torch(M, N, K):
+def test_bf16_gemm_matches_torch(M, N, K, ks, deferred):
     g = torch.Generator(device="cpu").manual_seed(M + N + K)
     lda = (K + 7) // 8 * 8
     A = torch.zeros((M, lda), dtype=torch.bfloat16)
@@ -36,7 +50,7 @@ def test_bf16_gemm_matches_torch(M, N, K):
     A[:, K:] = 7.0
     Bt[:, K:] = 7.0
     Ad, Bd = A.cuda(), Bt.cuda()
-    D = run_bf16(Ad, Bd, M, N, K)
+    D = run_bf16(Ad, Bd, M, N, K, ks, deferred)
     torch.cuda.synchronize()
     ref = (Bt[:, :K].double() @ A[:, :K].double().T)  # (N, M)
     scale = (Bt[:, :K].double().abs() @ A[:, :K].double().abs().T).clamp_min(1e-30)
@@ -44,8 +58,9 @@ def test_bf16_gemm_matches_torch(M, N, K):
     assert rel < 1e-5, rel
 
 
+@pytest.mark.parametrize("ks,deferred", SPLITS)
 @pytest.mark.parametrize("M,N,K", [(16, 16, 4), (64, 64, 16), (300, 70, 201), (1000, 257, 999)])
-def test_f64_gemm_matches_torch(M, N, K):
+def test_f64_gemm_matches_torch(M, N, K, ks, deferred):
     g = torch.Generator(device="cpu").manual_seed(M * 3 + N + K)
     lda = (K + 1) // 2 * 2
     A = torch.zeros((M, lda), dtype=torch.float64)
@@ -56,8 +71,10 @@ def test_f64_gemm_matches_torch(M, N, K):
     Bt[:, K:] = 7.0
     Ad, Bd = A.cuda(), Bt.cuda()
     D = torch.empty((N, M), dtype=torch.float64, device="cuda")
+    w = workspace(1, M, N, ks) if deferred else None
     check(lib().rwig_gemm_tn_f64_dev(M, N, K, Ad.data_ptr(), lda, Bd.data_ptr(), lda,
-                                     D.data_ptr(), M, torch.cuda.current_stream().cuda_stream))
+                                     D.data_ptr(), M, ks, None if w is None else w.data_ptr(),
+                                     torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     ref = Bt[:, :K] @ A[:, :K].T
     scale = (Bt[:, :K].abs() @ A[:, :K].abs().T).clamp_min(1e-300)
@@ -65,17 +82,19 @@ def test_f64_gemm_matches_torch(M, N, K):
     assert rel < 1e-13, rel
 
 
-def test_bf16_gemm_throughput_report():
+@pytest.mark.parametrize("ks,deferred", [(1, False), (2, True)])
+def test_bf16_gemm_throughput_report(ks, deferred):
     M, N, K = 4096, 1024, 4096
     A = torch.randn((M, K), device="cuda").to(torch.bfloat16)
     Bt = torch.randn((N, K), device="cuda").to(torch.bfloat16)
-    D = run_bf16(A, Bt, M, N, K)
+    D = run_bf16(A, Bt, M, N, K, ks, deferred)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(20):
-        run_bf16(A, Bt, M, N, K)
+        run_bf16(A, Bt, M, N, K, ks, deferred)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / 20
-    print(f"bf16 GEMM {M}x{N}x{K}: {dt*1e6:.1f} us, {2*M*N*K/dt/1e12:.1f} TFLOP/s", flush=True)
+    print(f"bf16 GEMM {M}x{N}x{K} ks={ks} deferred={deferred}: {dt*1e6:.1f} us, "
+          f"{2*M*N*K/dt/1e12:.1f} TFLOP/s", flush=True)
     ref = (Bt[:64].float() @ A.float().T)
     assert torch.allclose(D[:64], ref, rtol=1e-2, atol=1e-1)

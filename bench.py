@@ -35,7 +35,7 @@ METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TF = 2500.0
 ABSORB_MFMA = 30  # iterations between FP64 absorptions on the MFMA path
-GEMM_KSPLIT = 2   # k split of the MFMA path's BF16 products
+GEMM_KSPLIT = 1   # k split of the MFMA path's BF16 products (deferred epilogue)
 
 
 def pmc_traffic():
@@ -57,7 +57,7 @@ def log(msg):
 def c2_gemm(torch, dev, stream):
     """The dominant kernel of the MFMA path at its full shape: K^T W with
     M = K = 4096 (the cost's points) and N = 1024 (the targets), BF16 inputs,
-    k split in two, deferred epilogue. Returns (launch, flops per launch)."""
+    deferred epilogue. Returns (launch, flops per launch)."""
     from rwig_amd._lib import check, lib
     gM, gN, gK = 4096, 1024, 4096
     Wt = torch.randn((gN, gK), device=dev).to(torch.bfloat16)
@@ -86,7 +86,7 @@ def gemm_only():
     for _ in range(5):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print("gemm-only: 5 products 4096 x 1024 x 4096 (k split 2), 5 copies of 256 MiB",
+    print("gemm-only: 5 products 4096 x 1024 x 4096, 5 copies of 256 MiB",
           flush=True)
 
 
@@ -433,7 +433,7 @@ def main():
                                        "(profiles/c2_gemm_hbm_traffic.md)",
                      "alg_bytes_per_launch": (4096 * 4096 + 1024 * 4096) * 2 + 4096 * 1024 * 4,
                      "kernel": "k_gemm_tn<bf16> + k_gemm_splitk_epilogue (K^T W, 4096 x 1024 x "
-                               "4096, k split 2, as in the MFMA path)",
+                               "4096, deferred epilogue, as in the MFMA path)",
                      "kernel_ms": gemm_ms, "alg_flops_per_launch": gemm_flops},
         "cpu_baseline": cpu,
         **extra,

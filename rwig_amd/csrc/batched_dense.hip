@@ -166,11 +166,11 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   double *Bd = alloc_into<double>(pool, (size_t)ldn * B);
   float *Yc = alloc_into<float>(pool, MB);
   // deferred GEMM epilogues (the fused updates run in their own kernel, with
-  // the occupancy their loads need); the BF16 products split k in two, which
-  // gives two workgroups per CU at 1024 targets (measured: 13.6k vs 11.3k
-  // solves/s without the split)
+  // the occupancy their loads need); no k split: with 8-wave workgroups and two
+  // k-steps of loads in flight, one workgroup per CU hides the load latency
+  // (measured at 1024 targets: 16.0k solves/s, 15.6k with a split in two)
   SplitK<uint16_t> skb;
-  skb.ks = 2;
+  skb.ks = 1;
   skb.work = alloc_into<float>(
       pool, std::max(splitk_work_elems<uint16_t>(M, B, skb.ks), splitk_work_elems<uint16_t>(Nc, B, skb.ks)));
   SplitK<double> skd;

@@ -80,16 +80,27 @@ template <> struct Mfma<float> {
   __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// Tile shape of a product: workgroup tile BM x BN over k-steps of BK, LDS rows
+// padded by PAD elements, WGM x WGN wavefronts each computing a
+// (BM/WGM) x (BN/WGN) block of 16 x 16 MFMA tiles, PF k-steps of global loads
+// in flight (PF = 2: a second register set, so that a load has two k-steps of
+// MFMA work to arrive in).
+template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1>
+struct GemmShape {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WGM = WGM_, WGN = WGN_;
+  static constexpr int PF = PF_;  // k-steps of global loads in flight (1 or 2)
+  static constexpr int NT = 64 * WGM * WGN;  // threads per workgroup
+  static constexpr int WM = BM / WGM, WN = BN / WGN;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int LDL = BK + PAD;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave block must be a multiple of 16");
+  static_assert(BK % Mfma<T>::KI == 0, "k-step must be a multiple of the MFMA k");
+};
+
 template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> {
-  static constexpr int BM = 128, BN = 128, BK = 16, PAD = 4;
-};
-template <> struct GemmCfg<uint16_t> {
-  static constexpr int BM = 128, BN = 128, BK = 64, PAD = 8;
-};
-template <> struct GemmCfg<double> {
-  static constexpr int BM = 64, BN = 64, BK = 16, PAD = 2;
-};
+template <> struct GemmCfg<float> : GemmShape<float, 128, 128, 16, 4, 2, 2> {};
+template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 2, 2> {};
+template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
 
 // Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix
 // stored with its k values contiguous (element (r, k) at p[r * ld + k]) into
@@ -99,29 +110,47 @@ struct TileLoader {
   static constexpr int VEC = 16 / sizeof(T);
   static constexpr int CHUNKS = ROWS * BK / VEC;      // 16-byte chunks per tile
   static constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
+  static constexpr bool EVEN = CHUNKS % NT == 0;      // every thread has PER_THREAD chunks
   uint4 reg[PER_THREAD];
-  const T *row_ptr[PER_THREAD];  // start of this thread's rows (null: outside)
+  const T *row_ptr[PER_THREAD];  // start of this thread's rows (a valid row if outside)
+  unsigned inside = 0;           // bit c: chunk c lies on a row of the matrix
+
+  __device__ static bool has(int idx) { return EVEN || idx < CHUNKS; }
 
   // rows r0.. of a matrix with contiguous k (element (r, k) at p[r * ld + k]);
   // rmap (optional): tile row r reads stored row rmap[r]. Resolved once.
+  // Chunks outside the matrix read its first row and are zeroed, so that the
+  // loads of a k-step carry no branch (a branch around them makes the
+  // compiler wait for each load at once instead of after the MFMA work).
   __device__ void init(const T *__restrict__ p, long ld, int nrows, int r0,
                        const int *__restrict__ rmap) {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
       const int r = r0 + idx / (BK / VEC);
-      row_ptr[c] = nullptr;
-      if (idx < CHUNKS && r < nrows) row_ptr[c] = p + (rmap ? (long)rmap[r] : (long)r) * ld;
+      const bool in = has(idx) && r < nrows;
+      if (in) inside |= 1u << c;
+      row_ptr[c] = p + (in ? (rmap ? (long)rmap[r] : (long)r) * ld : 0L);
     }
   }
-  // loads the tile's k values k0 .. k0 + BK (k >= K read as zero)
-  __device__ void load(int K, int k0) {
+  // loads k values k0 .. k0 + BK of the tile's rows; needs k0 + BK <= K
+  __device__ void load_full(int k0) {
+#pragma unroll
+    for (int c = 0; c < PER_THREAD; ++c) {
+      const int idx = threadIdx.x + c * NT;
+      if (!has(idx)) continue;
+      const int k = k0 + (idx % (BK / VEC)) * VEC;
+      reg[c] = *reinterpret_cast<const uint4 *>(row_ptr[c] + k);  // zeroed in store()
+    }
+  }
+  // the last, partial k-step: k >= K read as zero
+  __device__ void load_tail(int K, int k0) {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
       const int k = k0 + (idx % (BK / VEC)) * VEC;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (row_ptr[c]) {
+      if (has(idx) && ((inside >> c) & 1u)) {
         const T *src = row_ptr[c] + k;
         if (k + VEC <= K) {
           v = *reinterpret_cast<const uint4 *>(src);
@@ -135,13 +164,24 @@ struct TileLoader {
       reg[c] = v;
     }
   }
+  __device__ void load(int K, int k0) {
+    if (k0 + BK <= K)
+      load_full(k0);
+    else
+      load_tail(K, k0);
+  }
+  // the loaded chunks into the LDS tile; chunks outside the matrix as zero
+  // (applied here, after the k-step's MFMA work, not at the load)
   __device__ void store(T *lds, int ldl) const {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
-      if (idx < CHUNKS) {
+      if (has(idx)) {
         const int row = idx / (BK / VEC), kc = (idx % (BK / VEC)) * VEC;
-        *reinterpret_cast<uint4 *>(lds + row * ldl + kc) = reg[c];
+        const bool in = (inside >> c) & 1u;
+        const uint4 v = reg[c];
+        *reinterpret_cast<uint4 *>(lds + row * ldl + kc) =
+            make_uint4(in ? v.x : 0u, in ? v.y : 0u, in ? v.z : 0u, in ? v.w : 0u);
       }
     }
   }
@@ -158,18 +198,13 @@ template <typename T> struct SplitK {
   typename Mfma<T>::out_t *work = nullptr;  // ks x tiles x (per-thread tile) x 256
 };
 
-// Accumulator elements per thread of one workgroup tile.
-template <typename T> constexpr int gemm_acc_per_thread() {
-  return (GemmCfg<T>::BM / 2 / 16) * (GemmCfg<T>::BN / 2 / 16) * 4;
-}
-
 // k splits for an M x K operand: enough workgroups along M x splits to cover
 // the chip when M alone gives few tiles, at least 4 k-steps per split. A
 // function of (M, K) only, so that the arithmetic of a column does not depend
 // on how many columns the product has.
-template <typename T> inline int splitk_for(int M, int K) {
-  const int tiles = (M + GemmCfg<T>::BM - 1) / GemmCfg<T>::BM;
-  const int steps = (K + GemmCfg<T>::BK - 1) / GemmCfg<T>::BK;
+template <typename T, class CF = GemmCfg<T>> inline int splitk_for(int M, int K) {
+  const int tiles = (M + CF::BM - 1) / CF::BM;
+  const int steps = (K + CF::BK - 1) / CF::BK;
   int ks = 256 / tiles;
   ks = std::min(ks, 8);
   ks = std::min(ks, steps / 4);
@@ -178,24 +213,23 @@ template <typename T> inline int splitk_for(int M, int K) {
 
 // Workspace elements of a split product with M rows, up to N columns and ks
 // splits.
-template <typename T> inline size_t splitk_work_elems(int M, int N, int ks) {
-  const size_t tiles = (size_t)((M + GemmCfg<T>::BM - 1) / GemmCfg<T>::BM) *
-                       ((N + GemmCfg<T>::BN - 1) / GemmCfg<T>::BN);
-  return (size_t)ks * tiles * gemm_acc_per_thread<T>() * 256;
+template <typename T, class CF = GemmCfg<T>>
+inline size_t splitk_work_elems(int M, int N, int ks) {
+  const size_t tiles = (size_t)((M + CF::BM - 1) / CF::BM) * ((N + CF::BN - 1) / CF::BN);
+  return (size_t)ks * tiles * CF::BM * CF::BN;
 }
 
 // Epilogue of one workgroup tile: op(m, n or nmap[n], value) for the tile's
 // elements inside M x N; accumulator lane layout: column lane % 16, row
 // Mfma::row(lane, r).
-template <typename T, class Op>
+template <typename T, class CF, class Op>
 __device__ __forceinline__ void gemm_epilogue(
     int M, int N, int m0, int n0, const int *__restrict__ nmap,
-    const typename Mfma<T>::acc_t (&acc)[GemmCfg<T>::BM / 32][GemmCfg<T>::BN / 32], Op &op) {
+    const typename Mfma<T>::acc_t (&acc)[CF::TM][CF::TN], Op &op) {
   using MF = Mfma<T>;
-  constexpr int WM = GemmCfg<T>::BM / 2, WN = GemmCfg<T>::BN / 2;
-  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int WM = CF::WM, WN = CF::WN, TM = CF::TM, TN = CF::TN;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wm = wv & 1, wn = wv >> 1, lr = lane & 15;
+  const int wm = wv % CF::WGM, wn = wv / CF::WGM, lr = lane & 15;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -211,28 +245,27 @@ __device__ __forceinline__ void gemm_epilogue(
     }
 }
 
-template <typename T, class Op>
-__global__ void __launch_bounds__(256)
+template <typename T, class CF, class Op>
+__global__ void __launch_bounds__(CF::NT)
 k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
           long ldb, const int *__restrict__ nmap, const int *__restrict__ ncount, SplitK<T> sk,
           Op op) {
   if (ncount) {  // column count read on the device (the grid covers an upper bound)
     const int nc = *ncount;
-    if ((int)blockIdx.y * GemmCfg<T>::BN >= nc) return;
+    if ((int)blockIdx.y * CF::BN >= nc) return;
     N = nc;
   }
   using MF = Mfma<T>;
-  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN, BK = GemmCfg<T>::BK;
-  constexpr int LDL = BK + GemmCfg<T>::PAD;
-  constexpr int WM = BM / 2, WN = BN / 2;  // per-wave block
-  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, LDL = CF::LDL;
+  constexpr int WM = CF::WM, WN = CF::WN;  // per-wave block
+  constexpr int TM = CF::TM, TN = CF::TN;
   // two LDS stages: the next k-step is stored into one while the other is
   // being multiplied, one barrier per k-step
   __shared__ __attribute__((aligned(16))) T As[2][BM * LDL];
   __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDL];
 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wm = wv & 1, wn = wv >> 1;
+  const int wm = wv % CF::WGM, wn = wv / CF::WGM;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 
   typename MF::acc_t acc[TM][TN];
@@ -241,28 +274,17 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
 
-  TileLoader<T, BM, BK, 256> la;
-  TileLoader<T, BN, BK, 256> lb;
+  TileLoader<T, BM, BK, CF::NT> la;
+  TileLoader<T, BN, BK, CF::NT> lb;
   // k range of this split (multiples of BK)
   const int steps = (K + BK - 1) / BK;
   const int per = (steps + sk.ks - 1) / sk.ks;
   const int kbeg = blockIdx.z * per * BK, kend = min(K, ((int)blockIdx.z + 1) * per * BK);
   la.init(A, lda, M, m0, nullptr);
   lb.init(B, ldb, N, n0, nmap);
-  la.load(K, kbeg);
-  lb.load(K, kbeg);
-  la.store(As[0], LDL);
-  lb.store(Bs[0], LDL);
-  __syncthreads();
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
-  int cur = 0;
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool more = k0 + BK < kend;
-    if (more) {
-      la.load(K, k0 + BK);
-      lb.load(K, k0 + BK);
-    }
-    const T *as = As[cur], *bs = Bs[cur];
+  // one k-step of MFMA work on an LDS stage
+  auto mma_step = [&](const T *as, const T *bs) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += MF::KI) {
       typename MF::frag_t af[TM], bfr[TN];
@@ -275,12 +297,62 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 #pragma unroll
         for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
     }
-    if (more) {
-      la.store(As[cur ^ 1], LDL);
-      lb.store(Bs[cur ^ 1], LDL);
+  };
+  // global loads of the k-step at kk0 into a register set: one uniform branch
+  // around both loaders, so that the full-step loads form one block with no
+  // join between them and the compiler defers their waits to the LDS stores
+  // that follow the MFMA work
+  auto issue = [&](TileLoader<T, BM, BK, CF::NT> &a, TileLoader<T, BN, BK, CF::NT> &b, int kk0) {
+    if (kk0 + BK <= K) {
+      a.load_full(kk0);
+      b.load_full(kk0);
+    } else {
+      a.load_tail(K, kk0);
+      b.load_tail(K, kk0);
     }
+  };
+  issue(la, lb, kbeg);
+  la.store(As[0], LDL);
+  lb.store(Bs[0], LDL);
+  if constexpr (CF::PF == 1) {
     __syncthreads();
-    cur ^= 1;
+    int cur = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) issue(la, lb, k0 + BK);
+      mma_step(As[cur], Bs[cur]);
+      if (more) {
+        la.store(As[cur ^ 1], LDL);
+        lb.store(Bs[cur ^ 1], LDL);
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+  } else {
+    // two register sets: at step s, LDS stage s % 2 holds step s, the other
+    // set's loads of step s + 1 are in flight, and this set takes step s + 2
+    TileLoader<T, BM, BK, CF::NT> la2 = la;
+    TileLoader<T, BN, BK, CF::NT> lb2 = lb;
+    const int nst = (kend - kbeg + BK - 1) / BK;
+    if (nst > 1) issue(la2, lb2, kbeg + BK);
+    __syncthreads();
+    for (int s = 0; s < nst; s += 2) {
+      if (s + 2 < nst) issue(la, lb, kbeg + (s + 2) * BK);
+      mma_step(As[0], Bs[0]);
+      if (s + 1 < nst) {
+        la2.store(As[1], LDL);
+        lb2.store(Bs[1], LDL);
+      }
+      __syncthreads();
+      if (s + 1 >= nst) break;
+      if (s + 3 < nst) issue(la2, lb2, kbeg + (s + 3) * BK);
+      mma_step(As[1], Bs[1]);
+      if (s + 2 < nst) {
+        la.store(As[0], LDL);
+        lb.store(Bs[0], LDL);
+      }
+      __syncthreads();
+    }
   }
   if (sk.work) {
     // partial tile of this split, column-major within the tile; the sum and
@@ -300,7 +372,7 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
         }
     return;
   }
-  gemm_epilogue<T>(M, N, m0, n0, nmap, acc, op);
+  gemm_epilogue<T, CF>(M, N, m0, n0, nmap, acc, op);
 }
 
 // Sum of the split partials of each tile, in split order, then the epilogue.
@@ -309,12 +381,13 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 // stores are contiguous across a wavefront.
 constexpr int kEpiCols = 16;
 
-template <typename T, class Op>
+template <typename T, class CF, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
                        const int *__restrict__ ncount, int ks,
                        const typename Mfma<T>::out_t *__restrict__ work, Op op) {
-  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
+  constexpr int BM = CF::BM, BN = CF::BN;
+  static_assert((BM * kEpiCols) % 256 == 0 && BN % kEpiCols == 0, "epilogue tiling");
   if (ncount) {
     const int nc = *ncount;
     if ((int)blockIdx.y * BN >= nc) return;
@@ -344,22 +417,22 @@ k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
 // of columns actually used, read by the kernel (N is then an upper bound that
 // sizes the grid), so that a launch can be recorded once in a graph and
 // replayed over a changing column set.
-template <typename T, class Op>
+template <typename T, class CF = GemmCfg<T>, class Op>
 void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
              Op op, const int *nmap = nullptr, const int *ncount = nullptr,
              const SplitK<T> &sk = SplitK<T>()) {
-  constexpr int BM = GemmCfg<T>::BM, BN = GemmCfg<T>::BN;
+  constexpr int BM = CF::BM, BN = CF::BN;
   RWIG_REQUIRE(M > 0 && N >= 0 && K > 0, "GEMM dimensions must be positive");
   RWIG_REQUIRE(lda % (16 / sizeof(T)) == 0 && ldb % (16 / sizeof(T)) == 0,
                "GEMM leading dimensions must be multiples of 16 bytes");
   RWIG_REQUIRE(sk.ks >= 1 && (sk.ks == 1 || sk.work), "split-k product needs its workspace");
   if (N == 0) return;
   const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, sk.ks);
-  hipLaunchKernelGGL((k_gemm_tn<T, Op>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, nmap,
-                     ncount, sk, op);
+  hipLaunchKernelGGL((k_gemm_tn<T, CF, Op>), grid, dim3(CF::NT), 0, st, M, N, K, A, lda, B,
+                     ldb, nmap, ncount, sk, op);
   check_launch();
   if (sk.work) {
-    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, Op>), dim3(grid.x, grid.y, BN / kEpiCols),
+    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, CF, Op>), dim3(grid.x, grid.y, BN / kEpiCols),
                        dim3(256), 0, st, M, N, nmap, ncount, sk.ks, sk.work, op);
     check_launch();
   }

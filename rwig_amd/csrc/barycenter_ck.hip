@@ -84,7 +84,12 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
   T *F = ar.make<T>(MS), *G = ar.make<T>(NS);
   T *logA = ar.make<T>(MS), *KV = ar.make<T>(MS), *logb = ar.make<T>(N);
   T *pm = ar.make<T>(lse_row_part_size(M, N, S)), *ps = ar.make<T>(lse_row_part_size(M, N, S));
+  // C^T for the column passes (row passes over C^T), and their partials
+  T *CT = ar.make<T>((size_t)M * N);
+  transpose<T>(st, C, M, N, M, CT, N);
+  T *cpm = ar.make<T>(col_t_part_size(M, N, S)), *cps = ar.make<T>(col_t_part_size(M, N, S));
   LoopState *ls = ar.make<LoopState>(1);
+  double *gpart = ar.make<double>(gram_part_size(S));
   const T treg = (T)reg;
   const int ncp = wg ? maxiter / ck + 1 : 0;
   T *Fc = wg ? ar.make<T>((size_t)ncp * MS) : nullptr;
@@ -139,8 +144,8 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         gate);
     // KTU = column soft-min of R(F, G); logb = -(G + KTU) w / reg;
     // G += reg logb 1^T + KTU
-    lse_col<T>(
-        st, C, M, N, M, F, M, G, N, S, reg,
+    lse_col_t<T>(
+        st, CT, M, N, N, F, M, G, N, S, reg, cpm, cps,
         [=] __device__(int j, const T *r) {
           T acc = T(0);
           for (int s = 0; s < S; ++s) acc += -(G[j + (long)s * N] + r[s]) * w[s];
@@ -181,7 +186,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
             const long k = i + (long)s * M;
             return -(double)KV[k] / reg - (double)logA[k];
           },
-          [=] __device__(double nrm) { finish_iteration_ck(gate, nrm); }, gate);
+          [=] __device__(double nrm) { finish_iteration_ck(gate, nrm); }, gate, gpart);
   };
 
   LoopState hs{};
@@ -245,7 +250,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
     T *hKV = ar.make<T>((size_t)(ck + 1) * MS), *hKTU = ar.make<T>((size_t)(ck + 1) * NS);
     T *Fbar = ar.make<T>(MS), *Gbar = ar.make<T>(NS), *lbb = ar.make<T>(N), *Y = ar.make<T>(NS);
     double *gwa = ar.make<double>(NS);
-    T *part = ar.make<T>(lse_row_part_size(M, N, S));
+    T *part = ar.make<T>(wrow_part_size(M, N, S)), *cpart = ar.make<T>(col_t_part_size(M, N, S));
     T *rlogA = ar.make<T>(MS);
     launch_map(st, MS, [=] __device__(long k) { rlogA[k] = (T)(reg * log(A[k])); });
     launch_map(st, MS, [=] __device__(long k) {
@@ -285,8 +290,8 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         const T *KVl = hKV + q * MS, *KTUl = hKTU + q * NS;
         if (l != L) {
           // Gbar(:,s) = -X_s^T Fbar(:,s); logbbar = reg * rowsum(Gbar)
-          wcol<T>(
-              st, C, M, N, M, Fl, M, Gl, N, S, reg, KVl, Fbar,
+          wcol_t<T>(
+              st, CT, M, N, N, Fl, M, Gl, N, S, reg, KVl, Fbar, cpart,
               [=] __device__(int j, const T *r) {
                 T acc = T(0);
                 for (int s = 0; s < S; ++s) {

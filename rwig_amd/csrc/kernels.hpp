@@ -171,67 +171,213 @@ void gemv_t(hipStream_t st, const double *K, int M, int N, long ldk,
 }
 
 // ---------------------------------------------------------------- soft-min passes
+//
+// The soft-min of R_s(i,j) = (C(i,j) - F(i,s)) - G(j,s) over i (columns) or
+// over j (rows), for all S slices in one sweep over C: each entry of C is read
+// once per pass whatever S is, and kept in a register while the S slices use
+// it. The log-sum-exp is accumulated online: a running minimum m and
+// sum t = sum exp(-(r - m)/reg), rescaled when the minimum drops, one exp per
+// entry:
+//   d = (r - m)/reg, e = exp(-|d|); d < 0: t = t e + 1, m = r; else t += e.
+// Two partial states merge as m = min(m1, m2),
+// t = t1 exp(-(m1 - m)/reg) + t2 exp(-(m2 - m)/reg). The result
+// m - reg log t is the two-pass form of src/sinkhorn_impl_log.cpp:48-50 up to
+// rounding.
 
-// Soft-min over rows i of R_s(i,j) = (C(i,j) - F(i,s)) - G(j,s), one
-// wavefront per column j, for every slice s:
-//   res[s] = min_i R - reg * log(sum_i exp(-(R - min_i R) / reg))
-// (the two-pass form of src/sinkhorn_impl_log.cpp:48-50).
+template <typename T> struct SoftMin {
+  T m, t;
+  __device__ void init() {
+    m = T(INFINITY);
+    t = T(0);
+  }
+  __device__ void add(T r, T inv_reg) {
+    const T d = (r - m) * inv_reg;
+    const T e = exp(-fabs(d));
+    t = (d < T(0)) ? fma(t, e, T(1)) : t + e;
+    m = fmin(m, r);
+  }
+  __device__ void merge(T m2, T t2, T inv_reg) {
+    const T mn = fmin(m, m2);
+    const T a = (m == mn) ? t : t * exp(-(m - mn) * inv_reg);
+    const T b = (m2 == mn) ? t2 : t2 * exp(-(m2 - mn) * inv_reg);
+    m = mn;
+    t = a + b;
+  }
+};
+
+template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int o) {
+  return __shfl_xor(v, o, kWave);
+}
+
+// Columns per workgroup of the column passes: each thread keeps CB x SM
+// partial states.
+template <typename T, int SM> struct ColBlock {
+  static constexpr int LIM = sizeof(T) == 4 ? 32 : 16;  // CB x SM states per thread
+  static constexpr int CB = (LIM / SM >= 8) ? 8 : (LIM / SM >= 1 ? LIM / SM : 1);
+};
+
+// Column soft-min: one workgroup per CB columns, threads along the rows.
+// op(j, res) with res[s] the soft-min of column j, slice s.
 template <typename T, int SM, class Op>
 __global__ void __launch_bounds__(kTB)
 k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
           const T *__restrict__ G, long ldg, int S, T reg, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (j >= N) return;
-  const T *col = C + (long)j * ldc;
-  T res[SM];
+  constexpr int CB = ColBlock<T, SM>::CB;
+  __shared__ T shm[kWavesPerBlock][CB * SM], sht[kWavesPerBlock][CB * SM];
+  const int j0 = blockIdx.x * CB;
+  const T inv = T(1) / reg;
+  T g[CB][SM];
+  SoftMin<T> st[CB][SM];
 #pragma unroll
-  for (int s = 0; s < SM; ++s) {
-    if (s < S) {
-      const T *f = F + (long)s * ldf;
-      const T gj = G[j + (long)s * ldg];
-      T mn = T(INFINITY);
-      for (int i = lane; i < M; i += kWave) mn = fmin(mn, (col[i] - f[i]) - gj);
-      mn = wave_min(mn);
-      T sm = T(0);
-      for (int i = lane; i < M; i += kWave) sm += exp(-(((col[i] - f[i]) - gj) - mn) / reg);
-      sm = wave_sum(sm);
-      res[s] = mn - reg * log(sm);
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      st[c][s].init();
+      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] : T(0);
+    }
+  for (int i = threadIdx.x; i < M; i += kTB) {
+    T f[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) f[s] = (s < S) ? F[i + (long)s * ldf] : T(0);
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      if (j0 + c >= N) continue;
+      const T x = C[i + (long)(j0 + c) * ldc];
+#pragma unroll
+      for (int s = 0; s < SM; ++s)
+        if (s < S) st[c][s].add((x - f[s]) - g[c][s], inv);
     }
   }
-  if (lane == 0) op(j, res);
+  // wavefront, then workgroup merge
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      if (s >= S) continue;
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1)
+        st[c][s].merge(shfl_xor_t(st[c][s].m, o), shfl_xor_t(st[c][s].t, o), inv);
+      if (lane == 0) {
+        shm[w][c * SM + s] = st[c][s].m;
+        sht[w][c * SM + s] = st[c][s].t;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x < CB) {
+    const int c = threadIdx.x;
+    if (j0 + c >= N) return;
+    T res[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      if (s >= S) continue;
+      SoftMin<T> a;
+      a.m = shm[0][c * SM + s];
+      a.t = sht[0][c * SM + s];
+      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][c * SM + s], sht[v][c * SM + s], inv);
+      res[s] = a.m - reg * log(a.t);
+    }
+    op(j0 + c, res);
+  }
 }
 
 template <typename T, class Op>
 void lse_col(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
              const T *G, long ldg, int S, double reg, Op op, const LoopState *ls) {
-  const dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
-  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_lse_col<T, SM, Op>), grid, dim3(kTB), 0, st, C,
-                                            M, N, ldc, F, ldf, G, ldg, S, (T)reg, op, ls));
+  RWIG_DISPATCH_S(S, SM, {
+    constexpr int CB = ColBlock<T, SM>::CB;
+    hipLaunchKernelGGL((k_lse_col<T, SM, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st, C, M,
+                       N, ldc, F, ldf, G, ldg, S, (T)reg, op, ls);
+  });
   check_launch();
 }
 
-// Row soft-min partials over the column chunk c: min and the shifted sum.
-template <typename T>
+// Row passes: a workgroup holds 64 rows (one per lane) and a chunk of the
+// columns, split between its 4 wavefronts; the wavefronts' partial states are
+// merged through LDS, one partial per (chunk, slice, row).
+constexpr int kRowsPerBlock = kWave;
+
+inline Chunking row_chunking(int M, int N) {
+  const int rowblocks = (M + kRowsPerBlock - 1) / kRowsPerBlock;
+  int nchunk = 2048 / (rowblocks > 0 ? rowblocks : 1);
+  const int maxc = (N + 4 * kWavesPerBlock - 1) / (4 * kWavesPerBlock);  // >= 4 columns per wave
+  if (nchunk > maxc) nchunk = maxc;
+  if (nchunk < 1) nchunk = 1;
+  const int cw = (N + nchunk - 1) / nchunk;
+  nchunk = (N + cw - 1) / cw;
+  return {nchunk, cw};
+}
+
+// Column tile of the row passes: the per-column values (G and, for the
+// adjoint, cshift and y) of kColTile columns are staged in LDS as
+// [column][slice], so that the wavefronts read them as broadcasts.
+constexpr int kColTile = 64;
+
+template <typename T, int SM>
+__device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, int t0, int nc,
+                                           int S, T *dst) {
+  for (int k = threadIdx.x; k < nc * SM; k += kTB) {
+    const int jj = k / SM, s = k % SM;
+    dst[k] = (s < S) ? src[(t0 + jj) + (long)s * ld] : T(0);
+  }
+}
+
+// Row soft-min partials over the column chunk blockIdx.y: pm = m, ps = t of
+// the chunk, at [(c * S + s) * M + i].
+template <typename T, int SM>
 __global__ void __launch_bounds__(kTB)
 k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
                long ldf, const T *__restrict__ G, long ldg, int S, T reg, int cw,
                T *__restrict__ pm, T *__restrict__ ps, const LoopState *ls) {
   if (ls && !ls->active) return;
-  const int i = blockIdx.x * kTB + threadIdx.x;
-  const int c = blockIdx.y, s = blockIdx.z;
-  if (i >= M) return;
+  __shared__ T shm[kWavesPerBlock][kRowsPerBlock], sht[kWavesPerBlock][kRowsPerBlock];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int i = blockIdx.x * kRowsPerBlock + lane;
+  const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const T fi = F[i + (long)s * ldf];
-  const T *g = G + (long)s * ldg;
-  T mn = T(INFINITY);
-  for (int j = j0; j < j1; ++j) mn = fmin(mn, (C[i + j * ldc] - fi) - g[j]);
-  T sm = T(0);
-  for (int j = j0; j < j1; ++j) sm += exp(-(((C[i + j * ldc] - fi) - g[j]) - mn) / reg);
-  const long o = ((long)c * S + s) * M + i;
-  pm[o] = mn;
-  ps[o] = sm;
+  const T inv = T(1) / reg;
+  const bool row = i < M;
+  T f[SM];
+  SoftMin<T> st[SM];
+#pragma unroll
+  for (int s = 0; s < SM; ++s) {
+    st[s].init();
+    f[s] = (row && s < S) ? F[i + (long)s * ldf] : T(0);
+  }
+  __shared__ T gs[kColTile * SM];
+  for (int t0 = j0; t0 < j1; t0 += kColTile) {
+    const int nc = min(kColTile, j1 - t0);
+    __syncthreads();
+    stage_cols<T, SM>(G, ldg, t0, nc, S, gs);
+    __syncthreads();
+    if (row)
+      for (int jj = w; jj < nc; jj += kWavesPerBlock) {
+        const T x = C[i + (long)(t0 + jj) * ldc];
+        const T *g = gs + jj * SM;
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+          if (s < S) st[s].add((x - f[s]) - g[s], inv);
+      }
+  }
+  // merge the wavefronts' states, one slice at a time through LDS
+#pragma unroll
+  for (int s = 0; s < SM; ++s) {
+    if (s >= S) break;
+    __syncthreads();
+    shm[w][lane] = st[s].m;
+    sht[w][lane] = st[s].t;
+    __syncthreads();
+    if (w == 0 && row) {
+      SoftMin<T> a;
+      a.m = shm[0][lane];
+      a.t = sht[0][lane];
+      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane], inv);
+      const long o = ((long)c * S + s) * M + i;
+      pm[o] = a.m;
+      ps[o] = a.t;
+    }
+  }
 }
 
 template <typename T, class Op>
@@ -239,16 +385,14 @@ __global__ void __launch_bounds__(kTB)
 k_lse_row_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, long len, T reg,
                 Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
+  const T inv = T(1) / reg;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
        k += (long)gridDim.x * blockDim.x) {
-    T m = pm[k];
-    for (int c = 1; c < nchunk; ++c) m = fmin(m, pm[(long)c * len + k]);
-    T t = T(0);
-    for (int c = 0; c < nchunk; ++c) {
-      const T mc = pm[(long)c * len + k];
-      t += (mc == m) ? ps[(long)c * len + k] : ps[(long)c * len + k] * exp(-(mc - m) / reg);
-    }
-    op(k, m - reg * log(t));
+    SoftMin<T> a;
+    a.m = pm[k];
+    a.t = ps[k];
+    for (int c = 1; c < nchunk; ++c) a.merge(pm[(long)c * len + k], ps[(long)c * len + k], inv);
+    op(k, a.m - reg * log(a.t));
   }
 }
 
@@ -258,10 +402,11 @@ template <typename T, class Op>
 void lse_row(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
              const T *G, long ldg, int S, double reg, T *pm, T *ps, Op op,
              const LoopState *ls) {
-  const Chunking ch = chunking(M, N);
-  dim3 grid((M + kTB - 1) / kTB, ch.nchunk, S);
-  hipLaunchKernelGGL(k_lse_row_part<T>, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S,
-                     (T)reg, ch.cw, pm, ps, ls);
+  const Chunking ch = row_chunking(M, N);
+  const dim3 grid((M + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
+  RWIG_DISPATCH_S(S, SM,
+                  hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, C, M, N,
+                                     ldc, F, ldf, G, ldg, S, (T)reg, ch.cw, pm, ps, ls));
   check_launch();
   const long len = (long)S * M;
   hipLaunchKernelGGL((k_lse_row_merge<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
@@ -270,7 +415,7 @@ void lse_row(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, lon
 }
 
 inline size_t lse_row_part_size(int M, int N, int S) {
-  return (size_t)chunking(M, N).nchunk * S * M;
+  return (size_t)row_chunking(M, N).nchunk * S * M;
 }
 
 // ---------------------------------------------------------------- soft-max weighted sums
@@ -278,41 +423,78 @@ inline size_t lse_row_part_size(int M, int N, int S) {
 // Column pass with row soft-max weights (the adjoint X of the row soft-min):
 //   res[s] = sum_i exp(-(R_s(i,j) - rshift(i,s)) / reg) * x(i,s)
 // where rshift is the row soft-min of R_s, so the weights of each row sum to 1.
+// One workgroup per CB columns, threads along the rows, C read once.
 template <typename T, int SM, class Op>
 __global__ void __launch_bounds__(kTB)
 k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
        const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ rshift,
        const T *__restrict__ x, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int j = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (j >= N) return;
-  const T *col = C + (long)j * ldc;
-  T res[SM];
+  constexpr int CB = ColBlock<T, SM>::CB;
+  __shared__ T sha[kWavesPerBlock][CB * SM];
+  const int j0 = blockIdx.x * CB;
+  const T inv = T(1) / reg;
+  T g[CB][SM], acc[CB][SM];
 #pragma unroll
-  for (int s = 0; s < SM; ++s) {
-    if (s < S) {
-      const T *f = F + (long)s * ldf;
-      const T *sh = rshift + (long)s * M;
-      const T *xs = x + (long)s * M;
-      const T gj = G[j + (long)s * ldg];
-      T acc = T(0);
-      for (int i = lane; i < M; i += kWave)
-        acc += exp(-(((col[i] - f[i]) - gj) - sh[i]) / reg) * xs[i];
-      res[s] = wave_sum(acc);
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      acc[c][s] = T(0);
+      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] : T(0);
+    }
+  for (int i = threadIdx.x; i < M; i += kTB) {
+    T f[SM], sh[SM], xv[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      f[s] = (s < S) ? F[i + (long)s * ldf] : T(0);
+      sh[s] = (s < S) ? rshift[i + (long)s * M] : T(0);
+      xv[s] = (s < S) ? x[i + (long)s * M] : T(0);
+    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      if (j0 + c >= N) continue;
+      const T cij = C[i + (long)(j0 + c) * ldc];
+#pragma unroll
+      for (int s = 0; s < SM; ++s)
+        if (s < S) acc[c][s] += exp(-(((cij - f[s]) - g[c][s]) - sh[s]) * inv) * xv[s];
     }
   }
-  if (lane == 0) op(j, res);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      if (s >= S) continue;
+      T v = acc[c][s];
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) v += shfl_xor_t(v, o);
+      if (lane == 0) sha[w][c * SM + s] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < CB) {
+    const int c = threadIdx.x;
+    if (j0 + c >= N) return;
+    T res[SM];
+#pragma unroll
+    for (int s = 0; s < SM; ++s) {
+      if (s >= S) continue;
+      T v = sha[0][c * SM + s];
+      for (int u = 1; u < kWavesPerBlock; ++u) v += sha[u][c * SM + s];
+      res[s] = v;
+    }
+    op(j0 + c, res);
+  }
 }
 
 template <typename T, class Op>
 void wcol(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf, const T *G,
           long ldg, int S, double reg, const T *rshift, const T *x, Op op,
           const LoopState *ls) {
-  const dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
-  RWIG_DISPATCH_S(S, SM, hipLaunchKernelGGL((k_wcol<T, SM, Op>), grid, dim3(kTB), 0, st, C, M,
-                                            N, ldc, F, ldf, G, ldg, S, (T)reg, rshift, x, op,
-                                            ls));
+  RWIG_DISPATCH_S(S, SM, {
+    constexpr int CB = ColBlock<T, SM>::CB;
+    hipLaunchKernelGGL((k_wcol<T, SM, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st, C, M, N,
+                       ldc, F, ldf, G, ldg, S, (T)reg, rshift, x, op, ls);
+  });
   check_launch();
 }
 
@@ -320,40 +502,205 @@ void wcol(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long l
 //   part = sum_{j in chunk} exp(-(R_s(i,j) - cshift(j,s) + rowsh(i,s)) / reg) * y(j,s)
 // rowsh (optional, M x S) scales row i by exp(-rowsh/reg) inside the
 // exponent, so rows whose result is tiny can be computed relative to their
-// own scale without underflow.
-template <typename T>
+// own scale without underflow. Layout as k_lse_row_part.
+template <typename T, int SM>
 __global__ void __launch_bounds__(kTB)
 k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
             const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ cshift,
             const T *__restrict__ y, const T *__restrict__ rowsh, int cw, T *__restrict__ part,
             const LoopState *ls) {
   if (ls && !ls->active) return;
-  const int i = blockIdx.x * kTB + threadIdx.x;
-  const int c = blockIdx.y, s = blockIdx.z;
-  if (i >= M) return;
+  __shared__ T sha[kWavesPerBlock][kRowsPerBlock];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int i = blockIdx.x * kRowsPerBlock + lane;
+  const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const T fi = rowsh ? F[i + (long)s * ldf] - rowsh[i + (long)s * M] : F[i + (long)s * ldf];
-  const T *g = G + (long)s * ldg;
-  const T *sh = cshift + (long)s * N;
-  const T *ys = y + (long)s * N;
-  T acc = T(0);
-  for (int j = j0; j < j1; ++j)
-    acc += exp(-(((C[i + j * ldc] - fi) - g[j]) - sh[j]) / reg) * ys[j];
-  part[((long)c * S + s) * M + i] = acc;
+  const T inv = T(1) / reg;
+  const bool row = i < M;
+  T f[SM], acc[SM];
+#pragma unroll
+  for (int s = 0; s < SM; ++s) {
+    acc[s] = T(0);
+    f[s] = T(0);
+    if (row && s < S) {
+      f[s] = F[i + (long)s * ldf];
+      if (rowsh) f[s] -= rowsh[i + (long)s * M];
+    }
+  }
+  __shared__ T gs[kColTile * SM], hs[kColTile * SM], ys[kColTile * SM];
+  for (int t0 = j0; t0 < j1; t0 += kColTile) {
+    const int nc = min(kColTile, j1 - t0);
+    __syncthreads();
+    stage_cols<T, SM>(G, ldg, t0, nc, S, gs);
+    stage_cols<T, SM>(cshift, N, t0, nc, S, hs);
+    stage_cols<T, SM>(y, N, t0, nc, S, ys);
+    __syncthreads();
+    if (row)
+      for (int jj = w; jj < nc; jj += kWavesPerBlock) {
+        const T x = C[i + (long)(t0 + jj) * ldc];
+        const T *g = gs + jj * SM, *h = hs + jj * SM, *yv = ys + jj * SM;
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+          if (s < S) acc[s] += exp(-(((x - f[s]) - g[s]) - h[s]) * inv) * yv[s];
+      }
+  }
+#pragma unroll
+  for (int s = 0; s < SM; ++s) {
+    if (s >= S) break;
+    __syncthreads();
+    sha[w][lane] = acc[s];
+    __syncthreads();
+    if (w == 0 && row) {
+      T v = sha[0][lane];
+      for (int u = 1; u < kWavesPerBlock; ++u) v += sha[u][lane];
+      part[((long)c * S + s) * M + i] = v;
+    }
+  }
+}
+
+inline size_t wrow_part_size(int M, int N, int S) {
+  return (size_t)row_chunking(M, N).nchunk * S * M;
 }
 
 template <typename T, class Op>
 void wrow(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf, const T *G,
           long ldg, int S, double reg, const T *cshift, const T *y, T *part, Op op,
           const LoopState *ls, const T *rowsh = nullptr) {
-  const Chunking ch = chunking(M, N);
-  dim3 grid((M + kTB - 1) / kTB, ch.nchunk, S);
-  hipLaunchKernelGGL(k_wrow_part<T>, grid, dim3(kTB), 0, st, C, M, N, ldc, F, ldf, G, ldg, S,
-                     (T)reg, cshift, y, rowsh, ch.cw, part, ls);
+  const Chunking ch = row_chunking(M, N);
+  const dim3 grid((M + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
+  RWIG_DISPATCH_S(S, SM,
+                  hipLaunchKernelGGL((k_wrow_part<T, SM>), grid, dim3(kTB), 0, st, C, M, N, ldc,
+                                     F, ldf, G, ldg, S, (T)reg, cshift, y, rowsh, ch.cw, part,
+                                     ls));
   check_launch();
   const long len = (long)S * M;
   hipLaunchKernelGGL((k_reduce_part<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
                      part, ch.nchunk, len, op, ls);
+  check_launch();
+}
+
+// ---------------------------------------------------------------- transpose
+
+// dst (cols x rows, leading dimension ldd) = src^T (src rows x cols, ld lds),
+// through 32 x 32 LDS tiles (reads and writes coalesced).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_transpose(const T *__restrict__ src, int rows, int cols, long lds, T *__restrict__ dst,
+            long ldd) {
+  __shared__ T tile[32][33];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + tx, c = c0 + k;
+    if (r < rows && c < cols) tile[k][tx] = src[r + (long)c * lds];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + tx, r = r0 + k;
+    if (r < rows && c < cols) dst[c + (long)r * ldd] = tile[tx][k];
+  }
+}
+
+template <typename T>
+void transpose(hipStream_t st, const T *src, int rows, int cols, long lds, T *dst, long ldd) {
+  const dim3 grid((rows + 31) / 32, (cols + 31) / 32);
+  hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, st, src, rows, cols, lds, dst, ldd);
+  check_launch();
+}
+
+// ---------------------------------------------------------------- column passes over C^T
+//
+// The column passes as row passes over CT = C^T (N x M, leading dimension
+// ldct), for callers that keep the transpose: the row kernels read CT
+// coalesced along its rows with the per-column values staged in LDS, where the
+// column kernels above reduce across all the threads of a column.
+// R_s(i,j) = (C(i,j) - G(j,s)) - F(i,s) here (the row kernels' order of the
+// two subtractions), equal to the column kernels' up to rounding.
+
+// res[s] of column j from the chunk partials of a row pass over CT. A block
+// covers 256 / SM columns: thread (column, slice) merges that slice's chunks
+// into LDS, then one thread per column calls op with the column's S values.
+template <typename T, int SM, class Op>
+__global__ void __launch_bounds__(kTB)
+k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, int N, int S,
+                T reg, Op op, const LoopState *ls) {
+  if (ls && !ls->active) return;
+  constexpr int JB = kTB / SM;
+  __shared__ T r[JB][SM];
+  const T inv = T(1) / reg;
+  const int jj = threadIdx.x / SM, s = threadIdx.x % SM;
+  const int j = blockIdx.x * JB + jj;
+  if (j < N && s < S) {
+    SoftMin<T> a;
+    const long k = (long)s * N + j, len = (long)S * N;
+    a.m = pm[k];
+    a.t = ps[k];
+    for (int c = 1; c < nchunk; ++c) a.merge(pm[c * len + k], ps[c * len + k], inv);
+    r[jj][s] = a.m - reg * log(a.t);
+  }
+  __syncthreads();
+  if (threadIdx.x < JB) {
+    const int jc = blockIdx.x * JB + threadIdx.x;
+    if (jc < N) op(jc, r[threadIdx.x]);
+  }
+}
+
+template <typename T, int SM, class Op>
+__global__ void __launch_bounds__(kTB)
+k_sum_cols(const T *__restrict__ part, int nchunk, int N, int S, Op op, const LoopState *ls) {
+  if (ls && !ls->active) return;
+  constexpr int JB = kTB / SM;
+  __shared__ T r[JB][SM];
+  const int jj = threadIdx.x / SM, s = threadIdx.x % SM;
+  const int j = blockIdx.x * JB + jj;
+  if (j < N && s < S) {
+    const long k = (long)s * N + j, len = (long)S * N;
+    T v = part[k];
+    for (int c = 1; c < nchunk; ++c) v += part[c * len + k];
+    r[jj][s] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < JB) {
+    const int jc = blockIdx.x * JB + threadIdx.x;
+    if (jc < N) op(jc, r[threadIdx.x]);
+  }
+}
+
+inline size_t col_t_part_size(int M, int N, int S) {
+  return (size_t)row_chunking(N, M).nchunk * S * N;
+}
+
+// column soft-min of C (as lse_col) from CT; pm, ps: col_t_part_size each
+template <typename T, class Op>
+void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F, long ldf,
+               const T *G, long ldg, int S, double reg, T *pm, T *ps, Op op,
+               const LoopState *ls) {
+  const Chunking ch = row_chunking(N, M);
+  const dim3 grid((N + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
+  RWIG_DISPATCH_S(S, SM, {
+    hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg,
+                       F, ldf, S, (T)reg, ch.cw, pm, ps, ls);
+    check_launch();
+    hipLaunchKernelGGL((k_lse_col_merge<T, SM, Op>), dim3((N + kTB / SM - 1) / (kTB / SM)),
+                       dim3(kTB), 0, st, pm, ps, ch.nchunk, N, S, (T)reg, op, ls);
+  });
+  check_launch();
+}
+
+// column weighted sums of C (as wcol) from CT; part: col_t_part_size
+template <typename T, class Op>
+void wcol_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F, long ldf,
+            const T *G, long ldg, int S, double reg, const T *rshift, const T *x, T *part, Op op,
+            const LoopState *ls) {
+  const Chunking ch = row_chunking(N, M);
+  const dim3 grid((N + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
+  RWIG_DISPATCH_S(S, SM, {
+    hipLaunchKernelGGL((k_wrow_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg, F,
+                       ldf, S, (T)reg, rshift, x, (const T *)nullptr, ch.cw, part, ls);
+    check_launch();
+    hipLaunchKernelGGL((k_sum_cols<T, SM, Op>), dim3((N + kTB / SM - 1) / (kTB / SM)), dim3(kTB),
+                       0, st, part, ch.nchunk, N, S, op, ls);
+  });
   check_launch();
 }
 
@@ -409,13 +756,18 @@ void big_sum(hipStream_t st, long n, F f, double *part /* kSumBlocks */, double 
 
 // Jacobi eigenvalue sweeps on the s x s symmetric matrix G (in place); returns
 // the largest eigenvalue. Same rotation sequence as the oracle's
-// spectral_norm().
+// spectral_norm(); the sweeps stop once the off-diagonal mass is below 1e-32 of
+// the diagonal's (the eigenvalues are then exact to rounding, the error being
+// quadratic in the off-diagonal part), where the oracle sweeps on to an exact
+// zero -- typically 100 sweeps that change nothing but the time.
 __device__ inline double jacobi_lmax(double *G, int s) {
   for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < s; ++p)
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < s; ++p) {
+      dia += G[p + p * s] * G[p + p * s];
       for (int q = p + 1; q < s; ++q) off += G[p + q * s] * G[p + q * s];
-    if (off == 0.0) break;
+    }
+    if (off <= 1e-32 * dia) break;
     for (int p = 0; p < s; ++p)
       for (int q = p + 1; q < s; ++q) {
         const double apq = G[p + q * s];
@@ -442,31 +794,140 @@ __device__ inline double jacobi_lmax(double *G, int s) {
 }
 
 // Spectral norm of the M x S matrix e(i, s) (Armadillo norm(X, 2),
-// src/barycenter_impl_log.cpp:252): Gram matrix by block sums, then Jacobi
-// on thread 0. fin(norm) runs on thread 0.
-template <class E, class Fin>
-__global__ void __launch_bounds__(1024)
-k_spectral(long M, int S, E e, Fin fin, const LoopState *gate) {
-  __shared__ double G[32 * 32];
-  __shared__ double sh[1024 / kWave];
-  if (gate && !gate->active) return;
-  for (int p = 0; p < S; ++p)
-    for (int q = p; q < S; ++q) {
-      double acc = 0.0;
-      for (long i = threadIdx.x; i < M; i += blockDim.x) acc += e(i, p) * e(i, q);
-      acc = block_sum(acc, sh);
-      if (threadIdx.x == 0) {
-        G[p + q * S] = acc;
-        G[q + p * S] = acc;
-      }
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) fin(sqrt(jacobi_lmax(G, S)));
+// src/barycenter_impl_log.cpp:252): the Gram matrix X^T X by partial sums over
+// row blocks (k_gram_part, kGramBlocks blocks, rows staged through LDS), then
+// one block adds the partials in block order and runs Jacobi on thread 0;
+// fin(norm) runs there.
+constexpr int kGramBlocks = 64;
+constexpr int kGramRows = 256;
+
+inline size_t gram_part_size(int S) { return (size_t)kGramBlocks * S * (S + 1) / 2; }
+
+__device__ inline void gram_pair(int k, int S, int &p, int &q) {
+  p = 0;
+  while (k >= S - p) {
+    k -= S - p;
+    ++p;
+  }
+  q = p + k;
 }
 
+template <class E>
+__global__ void __launch_bounds__(256)
+k_gram_part(long M, int S, E e, double *__restrict__ part, const LoopState *gate) {
+  if (gate && !gate->active) return;
+  __shared__ double tile[kGramRows * 32];
+  const int np = S * (S + 1) / 2;
+  constexpr int PR = (32 * 33 / 2 + 255) / 256;  // pairs per thread, S <= 32
+  double acc[PR];
+  int pp[PR], qq[PR];
+#pragma unroll
+  for (int r = 0; r < PR; ++r) {
+    acc[r] = 0.0;
+    pp[r] = qq[r] = 0;
+    const int k = threadIdx.x + 256 * r;
+    if (k < np) gram_pair(k, S, pp[r], qq[r]);
+  }
+  const long rows = (M + gridDim.x - 1) / gridDim.x;
+  const long i0 = blockIdx.x * rows, i1 = min(M, i0 + rows);
+  for (long t0 = i0; t0 < i1; t0 += kGramRows) {
+    const int nr = (int)min((long)kGramRows, i1 - t0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nr * S; idx += blockDim.x) {
+      const int rr = idx % nr, c = idx / nr;
+      tile[c * kGramRows + rr] = e(t0 + rr, c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+      if (threadIdx.x + 256 * r >= np) continue;
+      const double *xp = tile + pp[r] * kGramRows, *xq = tile + qq[r] * kGramRows;
+      double a = acc[r];
+      for (int rr = 0; rr < nr; ++rr) a += xp[rr] * xq[rr];
+      acc[r] = a;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PR; ++r) {
+    const int k = threadIdx.x + 256 * r;
+    if (k < np) part[(long)blockIdx.x * np + k] = acc[r];
+  }
+}
+
+// jacobi_lmax on a matrix of at most 8 x 8 held in registers (the loops are
+// unrolled; padding rows and columns are zero and skipped): the same rotations
+// without the LDS round trip of every element access.
+__device__ inline double jacobi_lmax8(const double *Gs, int s) {
+  constexpr int SM = 8;
+  double G[SM][SM];
+#pragma unroll
+  for (int p = 0; p < SM; ++p)
+#pragma unroll
+    for (int q = 0; q < SM; ++q) G[p][q] = (p < s && q < s) ? Gs[p + q * s] : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, dia = 0.0;
+#pragma unroll
+    for (int p = 0; p < SM; ++p) {
+      dia += G[p][p] * G[p][p];
+#pragma unroll
+      for (int q = p + 1; q < SM; ++q) off += G[p][q] * G[p][q];
+    }
+    if (off <= 1e-32 * dia) break;
+#pragma unroll
+    for (int p = 0; p < SM; ++p)
+#pragma unroll
+      for (int q = p + 1; q < SM; ++q) {
+        const double apq = G[p][q];
+        if (apq == 0.0) continue;
+        const double app = G[p][p], aqq = G[q][q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+        for (int k = 0; k < SM; ++k) {
+          const double gkp = G[k][p], gkq = G[k][q];
+          G[k][p] = c * gkp - sn * gkq;
+          G[k][q] = sn * gkp + c * gkq;
+        }
+#pragma unroll
+        for (int k = 0; k < SM; ++k) {
+          const double gpk = G[p][k], gqk = G[q][k];
+          G[p][k] = c * gpk - sn * gqk;
+          G[q][k] = sn * gpk + c * gqk;
+        }
+      }
+  }
+  double lmax = 0.0;
+#pragma unroll
+  for (int p = 0; p < SM; ++p) lmax = fmax(lmax, G[p][p]);
+  return lmax;
+}
+
+template <class Fin>
+__global__ void __launch_bounds__(256)
+k_gram_fin(int S, const double *__restrict__ part, Fin fin, const LoopState *gate) {
+  if (gate && !gate->active) return;
+  __shared__ double G[32 * 32];
+  const int np = S * (S + 1) / 2;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    double a = 0.0;
+    for (int b = 0; b < kGramBlocks; ++b) a += part[(long)b * np + k];
+    int p, q;
+    gram_pair(k, S, p, q);
+    G[p + q * S] = a;
+    G[q + p * S] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) fin(sqrt(S <= 8 ? jacobi_lmax8(G, S) : jacobi_lmax(G, S)));
+}
+
+// part: gram_part_size(S) doubles of device memory
 template <class E, class Fin>
-void spectral(hipStream_t st, long M, int S, E e, Fin fin, const LoopState *gate) {
-  hipLaunchKernelGGL((k_spectral<E, Fin>), dim3(1), dim3(1024), 0, st, M, S, e, fin, gate);
+void spectral(hipStream_t st, long M, int S, E e, Fin fin, const LoopState *gate, double *part) {
+  RWIG_REQUIRE(S >= 1 && S <= 32, "spectral norm of at most 32 columns");
+  hipLaunchKernelGGL((k_gram_part<E>), dim3(kGramBlocks), dim3(256), 0, st, M, S, e, part, gate);
+  check_launch();
+  hipLaunchKernelGGL((k_gram_fin<Fin>), dim3(1), dim3(256), 0, st, S, part, fin, gate);
   check_launch();
 }
 

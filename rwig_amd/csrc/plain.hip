@@ -396,7 +396,7 @@ SinkhornResult sinkhorn_log_dev(hipStream_t st, const double *a, const double *b
   if (wg) {
     const int Lf = fin.iter;
     DBuf<double> PbPb(MN), fbarb(M), gbarb(N), rsb(M), onesb(M > N ? M : N),
-        partb(gemv_n_part_size(M, N, 1));
+        partb(std::max(gemv_n_part_size(M, N, 1), wrow_part_size(M, N, 1)));
     double *PbP = PbPb.get(), *fbar = fbarb.get(), *gbar = gbarb.get(), *rs = rsb.get(),
            *ones = onesb.get(), *part = partb.get();
     launch_map(st, MN, [=] __device__(long k) { PbP[k] = (C[k] + reg * log(P[k])) * P[k]; });
@@ -617,8 +617,9 @@ SinkhornResult barycenter_log_dev(hipStream_t st, const double *A, const double 
   DBuf<double> logAb(MS), KVb(MS), KTUb(NS), logbb(N);
   DBuf<double> pmb(lse_row_part_size(M, N, S)), psb(lse_row_part_size(M, N, S));
   DBuf<LoopState> lsb(1);
+  DBuf<double> gpartb(gram_part_size(S));
   double *logA = logAb.get(), *KV = KVb.get(), *KTU = KTUb.get(), *logb = logbb.get(),
-         *pm = pmb.get(), *ps = psb.get();
+         *pm = pmb.get(), *ps = psb.get(), *gpart = gpartb.get();
   LoopState *ls = lsb.get();
   const int L = maxiter;
   DBuf<double> Fhb, Ghb, KVhb, KTUhb;
@@ -688,7 +689,7 @@ SinkhornResult barycenter_log_dev(hipStream_t st, const double *A, const double 
           const long k = i + (long)s * M;
           return -KV[k] / reg - logA[k];
         },
-        [=] __device__(double nrm) { finish_iteration(ls, nrm); }, ls);
+        [=] __device__(double nrm) { finish_iteration(ls, nrm); }, ls, gpart);
   };
   const LoopState fin = run_loop(st, ls, maxiter, iteration);
 
@@ -712,7 +713,7 @@ SinkhornResult barycenter_log_dev(hipStream_t st, const double *A, const double 
 
     const int Lf = fin.iter;
     DBuf<double> Fbarb(MS), Gbarb(NS), lbbb(N), Yb(NS), gwab(NS),
-        partb(gemv_n_part_size(M, N, S));
+        partb(std::max(gemv_n_part_size(M, N, S), wrow_part_size(M, N, S)));
     double *Fbar = Fbarb.get(), *Gbar = Gbarb.get(), *lbb = lbbb.get(), *Y = Yb.get(),
            *gwa = gwab.get(), *part = partb.get();
     fill(st, gA, MS, 0.0);

@@ -184,6 +184,14 @@ void gemv_t(hipStream_t st, const double *K, int M, int N, long ldk,
 // m - reg log t is the two-pass form of src/sinkhorn_impl_log.cpp:48-50 up to
 // rounding.
 
+// exp of the soft-min passes: FP64 keeps the library exp (the FP64 forms match
+// the oracle to 1e-10); FP32 uses the hardware exponential, v_exp_f32 on
+// x log2(e) (about 1 ulp, plus |x| ulp from the product: the arguments here
+// are <= 0 and terms below e^-87 vanish either way), a quarter of the
+// instructions of the range-reduced library form.
+__device__ __forceinline__ double sm_exp(double x) { return exp(x); }
+__device__ __forceinline__ float sm_exp(float x) { return __expf(x); }
+
 template <typename T> struct SoftMin {
   T m, t;
   __device__ void init() {
@@ -192,14 +200,14 @@ template <typename T> struct SoftMin {
   }
   __device__ void add(T r, T inv_reg) {
     const T d = (r - m) * inv_reg;
-    const T e = exp(-fabs(d));
+    const T e = sm_exp(-fabs(d));
     t = (d < T(0)) ? fma(t, e, T(1)) : t + e;
-    m = fmin(m, r);
+    m = (r < m) ? r : m;
   }
   __device__ void merge(T m2, T t2, T inv_reg) {
     const T mn = fmin(m, m2);
-    const T a = (m == mn) ? t : t * exp(-(m - mn) * inv_reg);
-    const T b = (m2 == mn) ? t2 : t2 * exp(-(m2 - mn) * inv_reg);
+    const T a = (m == mn) ? t : t * sm_exp(-(m - mn) * inv_reg);
+    const T b = (m2 == mn) ? t2 : t2 * sm_exp(-(m2 - mn) * inv_reg);
     m = mn;
     t = a + b;
   }
@@ -313,6 +321,7 @@ inline Chunking row_chunking(int M, int N) {
 // adjoint, cshift and y) of kColTile columns are staged in LDS as
 // [column][slice], so that the wavefronts read them as broadcasts.
 constexpr int kColTile = 64;
+constexpr int kColUnroll = 4;  // columns per wavefront whose loads are in flight together
 
 template <typename T, int SM>
 __device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, int t0, int nc,
@@ -324,7 +333,8 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, i
 }
 
 // Row soft-min partials over the column chunk blockIdx.y: pm = m, ps = t of
-// the chunk, at [(c * S + s) * M + i].
+// the chunk, at [(c * S + s) * M + i]. Slices s >= S (up to SM) run on zeros
+// and are not written, which keeps the inner loop free of per-slice branches.
 template <typename T, int SM>
 __global__ void __launch_bounds__(kTB)
 k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
@@ -351,14 +361,28 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
     __syncthreads();
     stage_cols<T, SM>(G, ldg, t0, nc, S, gs);
     __syncthreads();
-    if (row)
-      for (int jj = w; jj < nc; jj += kWavesPerBlock) {
+    if (row) {
+      // kColUnroll columns' loads issued ahead of their updates
+      int jj = w;
+      for (; jj + (kColUnroll - 1) * kWavesPerBlock < nc; jj += kColUnroll * kWavesPerBlock) {
+        T x[kColUnroll];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; ++u)
+          x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; ++u) {
+          const T *g = gs + (jj + u * kWavesPerBlock) * SM;
+#pragma unroll
+          for (int s = 0; s < SM; ++s) st[s].add((x[u] - f[s]) - g[s], inv);
+        }
+      }
+      for (; jj < nc; jj += kWavesPerBlock) {
         const T x = C[i + (long)(t0 + jj) * ldc];
         const T *g = gs + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s)
-          if (s < S) st[s].add((x - f[s]) - g[s], inv);
+        for (int s = 0; s < SM; ++s) st[s].add((x - f[s]) - g[s], inv);
       }
+    }
   }
   // merge the wavefronts' states, one slice at a time through LDS
 #pragma unroll
@@ -456,7 +480,7 @@ k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
       const T cij = C[i + (long)(j0 + c) * ldc];
 #pragma unroll
       for (int s = 0; s < SM; ++s)
-        if (s < S) acc[c][s] += exp(-(((cij - f[s]) - g[c][s]) - sh[s]) * inv) * xv[s];
+        if (s < S) acc[c][s] += sm_exp(-(((cij - f[s]) - g[c][s]) - sh[s]) * inv) * xv[s];
     }
   }
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -535,14 +559,28 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
     stage_cols<T, SM>(cshift, N, t0, nc, S, hs);
     stage_cols<T, SM>(y, N, t0, nc, S, ys);
     __syncthreads();
-    if (row)
-      for (int jj = w; jj < nc; jj += kWavesPerBlock) {
+    if (row) {
+      int jj = w;
+      for (; jj + (kColUnroll - 1) * kWavesPerBlock < nc; jj += kColUnroll * kWavesPerBlock) {
+        T x[kColUnroll];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; ++u)
+          x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; ++u) {
+          const int q = (jj + u * kWavesPerBlock) * SM;
+#pragma unroll
+          for (int s = 0; s < SM; ++s)
+            acc[s] += sm_exp(-(((x[u] - f[s]) - gs[q + s]) - hs[q + s]) * inv) * ys[q + s];
+        }
+      }
+      for (; jj < nc; jj += kWavesPerBlock) {
         const T x = C[i + (long)(t0 + jj) * ldc];
         const T *g = gs + jj * SM, *h = hs + jj * SM, *yv = ys + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s)
-          if (s < S) acc[s] += exp(-(((x - f[s]) - g[s]) - h[s]) * inv) * yv[s];
+        for (int s = 0; s < SM; ++s) acc[s] += sm_exp(-(((x - f[s]) - g[s]) - h[s]) * inv) * yv[s];
       }
+    }
   }
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
@@ -918,7 +956,24 @@ k_gram_fin(int S, const double *__restrict__ part, Fin fin, const LoopState *gat
     G[q + p * S] = a;
   }
   __syncthreads();
-  if (threadIdx.x == 0) fin(sqrt(S <= 8 ? jacobi_lmax8(G, S) : jacobi_lmax(G, S)));
+  if (threadIdx.x == 0) {
+    // With a non-positive tolerance the stopping test err >= tol of a
+    // non-final iteration depends only on err being a number: the eigenvalue
+    // sweeps run for the last iteration (whose err is reported) and whenever
+    // the tolerance can stop the loop; otherwise sqrt(trace) (an upper bound
+    // of the norm, NaN exactly when the matrix has one) decides the test.
+    const bool exact =
+        !gate || gate->tol > 0.0 || gate->iter + 1 >= gate->maxiter;
+    double v;
+    if (exact) {
+      v = sqrt(S <= 8 ? jacobi_lmax8(G, S) : jacobi_lmax(G, S));
+    } else {
+      double tr = 0.0;
+      for (int p = 0; p < S; ++p) tr += G[p + p * S];
+      v = sqrt(tr);
+    }
+    fin(v);
+  }
 }
 
 // part: gram_part_size(S) doubles of device memory

@@ -112,6 +112,52 @@ def c3_leg(log):
     return out
 
 
+def lse_leg(torch, dev, stream, log):
+    """The fused soft-min pass alone at the C3 shape (M = N = 4096, S = 8,
+    eps 0.004, FP32), row and column forms, timed with HIP events on their
+    stream: achieved HBM bandwidth of the algorithmic bytes (C once, F, G,
+    the output) against the 8 TB/s peak, and the exp rate (M N S exps)."""
+    from rwig_amd._lib import check, lib
+    M = N = 4096
+    S, reg = 8, 0.004
+    g = torch.Generator(device="cpu").manual_seed(1003)
+    C = torch.rand((N, M), generator=g).to(dev)            # column-major M x N
+    CT = C.t().contiguous()                                # column-major N x M
+    F = (0.01 * torch.randn((S, M), generator=g)).to(dev)  # column-major M x S
+    G = (0.01 * torch.randn((S, N), generator=g)).to(dev)
+    out_r = torch.empty((S, M), device=dev)
+    out_c = torch.empty((S, N), device=dev)
+    res = {"workload": f"soft-min pass M=N={M}, S={S}, eps={reg}, FP32 (C3 shape)",
+           "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    for which, name, ptr, ld, out in ((0, "rows", C, M, out_r), (1, "cols", CT, N, out_c)):
+        nb = ctypes.c_long()
+        check(lib().rwig_softmin_workspace_bytes(which, M, N, S, 1, ctypes.byref(nb)))
+        wk = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+        fn = lib().rwig_softmin_rows_dev if which == 0 else lib().rwig_softmin_cols_dev
+
+        def launch():
+            check(fn(ptr.data_ptr(), M, N, ld, F.data_ptr(), M, G.data_ptr(), N, S, reg,
+                     out.data_ptr(), 1, wk.data_ptr(), stream.cuda_stream))
+        launch()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 4 * (M * N + (M + N) * S + (M if which == 0 else N) * S)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS,
+                     "alg_bytes_per_launch": nbytes, "exps_per_launch": M * N * S,
+                     "gexp_per_s": M * N * S / (ms * 1e-3) / 1e9}
+        log(f"soft-min {name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s of C ({gbs / HBM_PEAK_GBS:.3f} "
+            f"of HBM peak), {M * N * S / (ms * 1e-3) / 1e9:.0f} G exp/s")
+    return res
+
+
 def c4_leg(torch, dev, stream, log):
     """WDL training step (V = 2000, K = 16 atoms, 512 documents, eps 0.1, L = 20):
     one batched gradient over all documents (the step's dominant work; the Adam
@@ -387,6 +433,7 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["c3_barycenter_gradient"] = c3_leg(log)
+        extra["lse_roofline"] = lse_leg(torch, dev, stream, log)
         extra["c4_wdl_step"] = c4_leg(torch, dev, stream, log)
         extra["c5_serving"] = c5_leg(log)
 

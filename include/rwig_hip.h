@@ -235,6 +235,25 @@ int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
                          const double *B, long ldb, double *D, long ldd,
                          int ks, void *work, void *stream);
 
+/* One fused soft-min pass on DEVICE pointers, the log-sum-exp reduction of
+ * the log-domain solvers (src/sinkhorn_impl_log.cpp:48-50 and its use in
+ * src/barycenter_impl_log.cpp:226-253), for S slices at once:
+ *   rows: out(i, s) = softmin_j [ (C(i,j) - F(i,s)) - G(j,s) ]      (M x S)
+ *   cols: out(j, s) = softmin_i [ (C(i,j) - G(j,s)) - F(i,s) ]      (N x S)
+ * with softmin_x r = min r - reg log sum exp(-(r - min r) / reg). The column
+ * pass reads CT = C^T (N x M, leading dimension ldct). precision 0: FP64,
+ * 1: FP32 (hardware exp). work: rwig_softmin_workspace_bytes(which = 0 rows,
+ * 1 cols, ...) bytes. Enqueued on the stream; no synchronisation. */
+int rwig_softmin_workspace_bytes(int which, int M, int N, int S, int precision,
+                                 long *bytes);
+int rwig_softmin_rows_dev(const void *C, int M, int N, long ldc, const void *F,
+                          long ldf, const void *G, long ldg, int S, double reg,
+                          void *out, int precision, void *work, void *stream);
+int rwig_softmin_cols_dev(const void *CT, int M, int N, long ldct,
+                          const void *F, long ldf, const void *G, long ldg,
+                          int S, double reg, void *out, int precision,
+                          void *work, void *stream);
+
 /* ---------------------------------------------------------------- serving
  * A serving session: continuous batching of Sinkhorn and barycenter solves
  * on cost matrices registered once and kept in HBM. No counterpart in the

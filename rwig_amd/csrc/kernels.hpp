@@ -226,12 +226,11 @@ template <typename T, int SM> struct ColBlock {
 
 // Column soft-min: one workgroup per CB columns, threads along the rows.
 // op(j, res) with res[s] the soft-min of column j, slice s.
-template <typename T, int SM, class Op>
+template <typename T, int SM, int CB, class Op>
 __global__ void __launch_bounds__(kTB)
 k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
           const T *__restrict__ G, long ldg, int S, T reg, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  constexpr int CB = ColBlock<T, SM>::CB;
   __shared__ T shm[kWavesPerBlock][CB * SM], sht[kWavesPerBlock][CB * SM];
   const int j0 = blockIdx.x * CB;
   const T inv = T(1) / reg;
@@ -290,13 +289,23 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
   }
 }
 
+// Few columns (fewer than 512 workgroups at CB per workgroup): one column per
+// workgroup, so that the pass still spreads over the CUs.
+template <typename T, int SM> inline bool col_block_one(int N) {
+  return ColBlock<T, SM>::CB > 1 && (N + ColBlock<T, SM>::CB - 1) / ColBlock<T, SM>::CB < 512;
+}
+
 template <typename T, class Op>
 void lse_col(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
              const T *G, long ldg, int S, double reg, Op op, const LoopState *ls) {
   RWIG_DISPATCH_S(S, SM, {
     constexpr int CB = ColBlock<T, SM>::CB;
-    hipLaunchKernelGGL((k_lse_col<T, SM, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st, C, M,
-                       N, ldc, F, ldf, G, ldg, S, (T)reg, op, ls);
+    if (col_block_one<T, SM>(N))
+      hipLaunchKernelGGL((k_lse_col<T, SM, 1, Op>), dim3(N), dim3(kTB), 0, st, C, M, N, ldc, F,
+                         ldf, G, ldg, S, (T)reg, op, ls);
+    else
+      hipLaunchKernelGGL((k_lse_col<T, SM, CB, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st,
+                         C, M, N, ldc, F, ldf, G, ldg, S, (T)reg, op, ls);
   });
   check_launch();
 }
@@ -448,13 +457,12 @@ inline size_t lse_row_part_size(int M, int N, int S) {
 //   res[s] = sum_i exp(-(R_s(i,j) - rshift(i,s)) / reg) * x(i,s)
 // where rshift is the row soft-min of R_s, so the weights of each row sum to 1.
 // One workgroup per CB columns, threads along the rows, C read once.
-template <typename T, int SM, class Op>
+template <typename T, int SM, int CB, class Op>
 __global__ void __launch_bounds__(kTB)
 k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
        const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ rshift,
        const T *__restrict__ x, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  constexpr int CB = ColBlock<T, SM>::CB;
   __shared__ T sha[kWavesPerBlock][CB * SM];
   const int j0 = blockIdx.x * CB;
   const T inv = T(1) / reg;
@@ -516,8 +524,12 @@ void wcol(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long l
           const LoopState *ls) {
   RWIG_DISPATCH_S(S, SM, {
     constexpr int CB = ColBlock<T, SM>::CB;
-    hipLaunchKernelGGL((k_wcol<T, SM, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st, C, M, N,
-                       ldc, F, ldf, G, ldg, S, (T)reg, rshift, x, op, ls);
+    if (col_block_one<T, SM>(N))
+      hipLaunchKernelGGL((k_wcol<T, SM, 1, Op>), dim3(N), dim3(kTB), 0, st, C, M, N, ldc, F, ldf,
+                         G, ldg, S, (T)reg, rshift, x, op, ls);
+    else
+      hipLaunchKernelGGL((k_wcol<T, SM, CB, Op>), dim3((N + CB - 1) / CB), dim3(kTB), 0, st, C,
+                         M, N, ldc, F, ldf, G, ldg, S, (T)reg, rshift, x, op, ls);
   });
   check_launch();
 }

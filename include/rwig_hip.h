@@ -235,6 +235,24 @@ int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda,
                          const double *B, long ldb, double *D, long ldd,
                          int ks, void *work, void *stream);
 
+/* Cost and histogram construction (src/utils.cpp:17-68).
+ * rwig_euclidean: the V x V matrix of Euclidean distances between the rows
+ * of E (V x d, column-major), as euclidean_cpp (src/utils.cpp:17-37);
+ * squared = 1 gives the squared distances (the cost the WDL configuration
+ * uses). Exactly symmetric with a zero diagonal. Host pointers;
+ * rwig_euclidean_dev takes device pointers and enqueues on the stream.
+ * rwig_doc2dist: documents as normalised bags of words over a dictionary, as
+ * doc2dist_cpp (src/utils.cpp:39-68): out (V x D, column-major, host) has in
+ * column c the counts of document c's tokens divided by their sum; a token
+ * maps to the first equal dictionary entry, or to the last entry when it is
+ * not in the dictionary (src/utils.cpp:53-57). Document c's tokens are
+ * tokens[offsets[c] .. offsets[c+1]), NUL-terminated strings. */
+int rwig_euclidean(const double *E, int V, int d, double *out, int squared);
+int rwig_euclidean_dev(const double *E, int V, int d, long lde, double *out,
+                       long ldo, int squared, void *stream);
+int rwig_doc2dist(const char *const *tokens, const long *offsets, int D,
+                  const char *const *dict, int V, double *out);
+
 /* One fused soft-min pass on DEVICE pointers, the log-sum-exp reduction of
  * the log-domain solvers (src/sinkhorn_impl_log.cpp:48-50 and its use in
  * src/barycenter_impl_log.cpp:226-253), for S slices at once:

@@ -229,3 +229,34 @@ def spectral_norm(X):
     if X.ndim == 1:
         X = X.reshape(-1, 1, order="F")
     return lib().oracle_spectral_norm(_ptr(X), X.shape[0], X.shape[1])
+
+
+def euclidean(E):
+    """euclidean_cpp (src/utils.cpp:17-37): D(i, j) = sqrt(sum_k (E(i,k) -
+    E(j,k))^2) for i < j, mirrored, zero diagonal. Pure Python loops over the
+    pairs (small inputs only)."""
+    E = np.asarray(E, dtype=np.float64)
+    V = E.shape[0]
+    D = np.zeros((V, V))
+    for i in range(V):
+        for j in range(i + 1, V):
+            c = float(np.sqrt(np.sum((E[i] - E[j]) ** 2)))
+            D[i, j] = c
+            D[j, i] = c
+    return D
+
+
+def doc2dist(docs, dictionary):
+    """doc2dist_cpp (src/utils.cpp:39-68): token -> index of the first equal
+    dictionary entry (std::find), the last index when absent (:53-57); counts
+    per document, each column divided by its sum (:61-63)."""
+    V = len(dictionary)
+    out = np.zeros((V, len(docs)))
+    for j, doc in enumerate(docs):
+        for tok in doc:
+            idx = dictionary.index(tok) if tok in dictionary else V
+            idx = V - 1 if idx == V else idx
+            out[idx, j] += 1.0
+        with np.errstate(invalid="ignore", divide="ignore"):
+            out[:, j] /= out[:, j].sum()
+    return out

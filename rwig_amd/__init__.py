@@ -4,5 +4,6 @@ sinkhorn() / barycenter() / wdl() entry points.
 The solves run in librwig_hip.so (hand-written HIP kernels for gfx950),
 reached through a C ABI (include/rwig_hip.h).
 """
-from .api import barycenter, barycenter_log_ck, cuda_available, sinkhorn, wdl  # noqa: F401
+from .api import (barycenter, barycenter_log_ck, cuda_available, doc2dist,  # noqa: F401
+                  euclidean, sinkhorn, wdl)
 from ._lib import RwigError  # noqa: F401

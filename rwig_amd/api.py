@@ -208,6 +208,39 @@ def barycenter_log_ck(A, C, w, b_ext=None, reg=0.1, with_grad=False, max_iter=10
     return out
 
 
+def euclidean(embedding, squared=False):
+    """Distance matrix between the rows of an embedding (V x d): mirrors
+    euclidean() (R/utils.R:4-7, euclidean_cpp src/utils.cpp:17-37);
+    squared=True gives squared distances."""
+    E = _f64(embedding)
+    if E.ndim != 2:
+        raise ValueError("embedding must be a numeric matrix!")
+    V, d = E.shape
+    out = np.zeros((V, V), order="F")
+    check(lib().rwig_euclidean(_p(E), V, d, _p(out), int(bool(squared))))
+    return out
+
+
+def doc2dist(doc_tokens, dictionary):
+    """Normalised bag-of-words matrix (V x D) of tokenised documents over a
+    dictionary: mirrors doc2dist() (R/utils.R:11-14, doc2dist_cpp
+    src/utils.cpp:39-68), including its rule for tokens not in the
+    dictionary (they count for the last entry)."""
+    docs = [list(d) for d in doc_tokens]
+    words = [str(w).encode() for w in dictionary]
+    if not words:
+        raise ValueError("the dictionary is empty")
+    toks = [str(t).encode() for d in docs for t in d]
+    offsets = np.zeros(len(docs) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([len(d) for d in docs])
+    tok_arr = (ctypes.c_char_p * max(1, len(toks)))(*toks) if toks else None
+    dict_arr = (ctypes.c_char_p * len(words))(*words)
+    out = np.zeros((len(words), len(docs)), order="F")
+    check(lib().rwig_doc2dist(tok_arr, offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                              len(docs), dict_arr, len(words), _p(out)))
+    return out
+
+
 def cuda_available():
     """cuda_available_cpp() (src/check_cuda.cpp:6-12): is a HIP device usable."""
     return bool(lib().rwig_available())

@@ -321,3 +321,29 @@ def test_wdl_end_to_end_runs_and_is_normalised():
     np.testing.assert_allclose(out["A"].sum(0), 1, rtol=1e-13)
     np.testing.assert_allclose(out["W"].sum(0), 1, rtol=1e-13)
     np.testing.assert_allclose(out["Yhat"].sum(0), 1, rtol=1e-12)
+
+
+# ------------------------------------------------------------- cost / histograms
+
+
+def test_oracle_euclidean_against_scipy():
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(11)
+    E = rng.standard_normal((23, 5))
+    D = O.euclidean(E)
+    assert np.allclose(D, cdist(E, E), rtol=1e-14, atol=1e-14)
+    assert np.array_equal(D, D.T) and np.all(np.diag(D) == 0.0)
+
+
+def test_oracle_doc2dist_known_answer():
+    # "zzz" is not in the dictionary: it counts for the last entry; "a"
+    # appears twice in the dictionary: the first one counts
+    docs = [["a", "b", "a"], ["c", "zzz"], ["b"]]
+    dictionary = ["a", "b", "c", "a"]
+    out = O.doc2dist(docs, dictionary)
+    expect = np.zeros((4, 3))
+    expect[0, 0], expect[1, 0] = 2 / 3, 1 / 3
+    # "zzz" goes to the last index, 3 (the duplicate "a"), "c" to index 2
+    expect[2, 1], expect[3, 1] = 0.5, 0.5
+    expect[1, 2] = 1.0
+    assert np.array_equal(out, expect)

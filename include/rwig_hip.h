@@ -253,6 +253,36 @@ int rwig_euclidean_dev(const double *E, int V, int d, long lde, double *out,
 int rwig_doc2dist(const char *const *tokens, const long *offsets, int D,
                   const char *const *dict, int V, double *out);
 
+/* Truncated SVD, replacing tsvd_cpp(M, k, flip_sign) (src/tsvd.cpp:7-86):
+ * out (m x kk, column-major, host) = U_k diag(s_k) for the kk = min(k, m, n)
+ * leading singular triplets of M (m x n, column-major, host), with the sign
+ * of each column set by flip_sign: 0 "auto" (the statistic of Bro, Acar &
+ * Kolda (2008) as src/tsvd.cpp:39-71 computes it), 1 "sklearn" (the entry of
+ * largest magnitude positive), 2 "none". k_out (optional) receives kk;
+ * residual (optional) the relative Ritz residual at which the subspace
+ * iteration stopped. k < 1 is an error (the reference's column range would
+ * be out of bounds). */
+int rwig_tsvd(double *out, int *k_out, double *residual, const double *M,
+              int m, int n, int k, int flip_sign);
+/* WIG document scores (R/wig.R:96-106) from the outputs of wdl: A (N x S
+ * topics), W (S x M weights), Yhat (N x M reconstructions), host,
+ * column-major. svd_method 0 ("topics"): scores = c(t(tsvd(t(A), 1)) %*% W);
+ * 1 ("docs"): scores = c(tsvd(t(Yhat), 1)); both with the "auto" sign rule.
+ * The unused matrix may be NULL. scores: M values. */
+int rwig_wig_scores(double *scores, const double *A, const double *W,
+                    const double *Yhat, int N, int S, int M, int svd_method);
+/* WIG index (R/wig.R:2-6, :139-159): the M document scores summed per
+ * calendar period of their dates (days since 1970-01-01, R's Date), periods
+ * in ascending order; with standardize, (x - mean(x)) / sd(x) + 100 with R's
+ * mean and sd (NaN for a single period, where R gives NA). group_unit (the
+ * unit of lubridate::floor_date): 0 day, 1 week (from Sunday), 2 month,
+ * 3 bimonth, 4 quarter, 5 halfyear, 6 year, 7 season (from December, March,
+ * June, September). period_days and index must hold M values; n_periods
+ * receives their count. Host only: no device work. */
+int rwig_wig_index(int *period_days, double *index, int *n_periods,
+                   const double *scores, const int *days, int M,
+                   int group_unit, int standardize);
+
 /* One fused soft-min pass on DEVICE pointers, the log-sum-exp reduction of
  * the log-domain solvers (src/sinkhorn_impl_log.cpp:48-50 and its use in
  * src/barycenter_impl_log.cpp:226-253), for S slices at once:

@@ -260,3 +260,123 @@ def doc2dist(docs, dictionary):
         with np.errstate(invalid="ignore", divide="ignore"):
             out[:, j] /= out[:, j].sum()
     return out
+
+
+def tsvd(M, k=1, flip_sign=0):
+    """tsvd_cpp (src/tsvd.cpp:7-86) on LAPACK's full SVD (numpy.linalg.svd,
+    gesdd: the divide-and-conquer routine Armadillo's svd() calls).
+
+    Returns dict(Msvd=U_k diag(s_k) (m x kk), sk_left, sk_right) where the
+    two lists hold the auto-mode statistics of :46-58 (zeros otherwise).
+    The auto statistic is taken on Y = M - U_k S_k V_k^T with all kk
+    components removed (:41), so u_K^T Y and Y v_K vanish up to rounding and
+    its sign is a property of the arithmetic, not of M."""
+    X = np.asarray(M, dtype=np.float64)
+    U, s, Vt = np.linalg.svd(X, full_matrices=True)
+    V = Vt.T.copy()
+    kk = min(int(k), s.size)                                   # :37
+    if kk < 1:
+        raise ValueError("k must be at least 1")
+    sk_l, sk_r = [0.0] * kk, [0.0] * kk
+    if flip_sign == 0:
+        Y = X - U[:, :kk] @ np.diag(s[:kk]) @ V[:, :kk].T      # :41
+        for K in range(kk):
+            sk_left = 0.0
+            for j in range(Y.shape[1]):                        # :48-52
+                uTy = float(np.sum(U[:, K] * Y[:, j]))
+                sk_left += np.sign(uTy) * uTy ** 2
+            sk_right = 0.0
+            for i in range(Y.shape[0]):                        # :54-58
+                vTy = float(np.sum(V[:, K] * Y[i, :]))
+                sk_right += np.sign(vTy) * vTy ** 2
+            if sk_left * sk_right < 0:                         # :60-66
+                if abs(sk_left) < abs(sk_right):
+                    sk_left = -sk_left
+                else:
+                    sk_right = -sk_right
+            U[:, K] *= np.sign(sk_left)                        # :68-69
+            V[:, K] *= np.sign(sk_right)
+            sk_l[K], sk_r[K] = sk_left, sk_right
+        Msvd = U[:, :kk] * s[:kk]                              # :71
+    elif flip_sign == 1:
+        Msvd = U[:, :kk] * s[:kk]                              # :74-80
+        for c in range(kk):
+            i = int(np.argmax(np.abs(Msvd[:, c])))
+            if Msvd[i, c] < 0:
+                Msvd[:, c] *= -1
+    else:
+        Msvd = U[:, :kk] * s[:kk]                              # :82
+    return {"Msvd": Msvd, "sk_left": sk_l, "sk_right": sk_r}
+
+
+def wig_scores(A, W, Yhat, svd_method="topics"):
+    """Document scores of wig.data.frame (R/wig.R:96-106): "topics" gives
+    c(t(tsvd(t(A), k = 1)) %*% W), "docs" gives c(tsvd(t(Yhat), k = 1)),
+    both with tsvd's default "auto" sign rule."""
+    if svd_method == "topics":
+        t = tsvd(np.asarray(A, dtype=np.float64).T, 1, 0)["Msvd"].T   # 1 x S
+        return (t @ np.asarray(W, dtype=np.float64)).ravel()
+    if svd_method == "docs":
+        return tsvd(np.asarray(Yhat, dtype=np.float64).T, 1, 0)["Msvd"].ravel()
+    raise ValueError("`svd_method` not implemented!")
+
+
+_UNITS = ("day", "week", "month", "bimonth", "quarter", "halfyear", "year", "season")
+
+
+def floor_period(day, unit):
+    """lubridate::floor_date of a Date (days since 1970-01-01) to the first
+    day of its period (R/wig.R:3), with Python's calendar (datetime.date):
+    weeks start on Sunday (lubridate's week_start = 7), seasons in December,
+    March, June and September."""
+    import datetime
+    epoch = datetime.date(1970, 1, 1)
+    dt = epoch + datetime.timedelta(days=int(day))
+    if unit == "day":
+        return int(day)
+    if unit == "week":
+        return int(day) - dt.isoweekday() % 7
+    if unit == "season":
+        y, m = dt.year, dt.month
+        start = datetime.date(y - (m <= 2), 12, 1) if m in (12, 1, 2) else \
+            datetime.date(y, 3 * (m // 3), 1)
+        return (start - epoch).days
+    span = {"month": 1, "bimonth": 2, "quarter": 3, "halfyear": 6, "year": 12}[unit]
+    start = datetime.date(dt.year, (dt.month - 1) // span * span + 1, 1)
+    return (start - epoch).days
+
+
+def wig_index(days, scores, group_unit="month", standardize=True):
+    """aggregate_by_period + standardisation of wig.data.frame (R/wig.R:2-6,
+    :139-159): scores summed per period in document order (R's sum() adds in
+    extended precision: numpy.longdouble here), periods ascending, then
+    (x - mean) / sd + 100 with R's two-pass mean and n - 1 sd (NaN for one
+    period). Returns (period start days, index)."""
+    sums = {}
+    for d, v in zip(days, scores):
+        key = floor_period(d, group_unit)
+        sums[key] = sums.get(key, np.longdouble(0.0)) + np.longdouble(float(v))
+    keys = sorted(sums)
+    x = np.array([float(sums[k]) for k in keys])
+    if standardize:
+        n = len(x)
+        s = np.longdouble(0.0)
+        for v in x:
+            s += np.longdouble(v)
+        s /= n
+        if np.isfinite(float(s)):
+            t = np.longdouble(0.0)
+            for v in x:
+                t += np.longdouble(v) - s
+            s += t / n
+        mean = float(s)
+        if n > 1:
+            q = np.longdouble(0.0)
+            for v in x:
+                dv = float(v) - mean
+                q += np.longdouble(dv * dv)
+            sd = float(np.sqrt(float(q / (n - 1))))
+        else:
+            sd = float("nan")
+        x = (x - mean) / sd + 100.0
+    return np.array(keys, dtype=np.int64), x

@@ -60,6 +60,9 @@ SIGNATURES = {
     "rwig_euclidean_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _i, _vp], _i),
     "rwig_doc2dist": ([ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_long), _i,
                        ctypes.POINTER(ctypes.c_char_p), _i, _dp], _i),
+    "rwig_tsvd": ([_dp, _ip, _dp, _dp, _i, _i, _i, _i], _i),
+    "rwig_wig_scores": ([_dp, _dp, _dp, _dp, _i, _i, _i, _i], _i),
+    "rwig_wig_index": ([_ip, _dp, _ip, _dp, _ip, _i, _i, _i], _i),
     "rwig_softmin_workspace_bytes": ([_i, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_long)], _i),
     "rwig_softmin_rows_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _vp,
                                ctypes.c_long, _i, _d, _vp, _i, _vp, _vp], _i),

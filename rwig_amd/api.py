@@ -259,7 +259,10 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
     With init=(Alpha0, Lambda0) the explicit starting parameters replace the
     seeded draw (the reference's CPU path draws them with R's RNG);
     precision "fp32" runs the Gibbs products and recurrences in FP32.
-    Only the parallel barycenter (sinkhorn_mode 1) is on the GPU path."""
+    sinkhorn_mode selects the inference of Yhat only, as in the reference
+    (src/wdl_impl.cpp:354-377): 1 batches the parallel barycenters of all
+    documents on the GPU; 2 runs one log-domain barycenter (FP64, GPU) per
+    document with the reference's spectral-norm stopping rule."""
     if sinkhorn_mode not in (1, 2):
         raise ValueError("Sinkhorn mode not supported")
     if isinstance(optimizer, str):
@@ -271,20 +274,18 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
     N, M = Y.shape
     if N <= S:
         raise ValueError("Number of topics S must be smaller than the vocab size N!")
-    if sinkhorn_mode != 1:
-        raise ValueError("the GPU WDL path infers Yhat with the parallel barycenter "
-                         "(sinkhorn_mode = 1)")
     if not usecuda:
         raise ValueError("this build runs every solve on the GPU; usecuda=False is not offered")
     A = np.zeros((N, S), order="F")
     W = np.zeros((S, M), order="F")
     Yhat = np.zeros((N, M), order="F")
+    Yp = _p(Yhat) if sinkhorn_mode == 1 else None  # mode 2: inferred below
     args = (float(reg), int(max_iter), float(zero_tol), int(batch_size), int(epochs),
             int(optimizer), float(eta), float(gamma), float(beta1), float(beta2), float(eps))
     if init is None:
         if precision != "fp64":
             raise ValueError("the seeded entry point runs in FP64")
-        check(lib().rwig_wdl(_p(A), _p(W), _p(Yhat), _p(Y), _p(C), N, M, int(S), *args,
+        check(lib().rwig_wdl(_p(A), _p(W), Yp, _p(Y), _p(C), N, M, int(S), *args,
                              int(seed), int(bool(verbose))))
     else:
         Alpha0 = _f64(init[0])
@@ -292,6 +293,10 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
         if Alpha0.shape != (N, S) or Lambda0.shape != (S, M):
             raise ValueError("init must be (Alpha0 N x S, Lambda0 S x M)")
         prec = {"fp64": 0, "fp32": 1}[precision]
-        check(lib().rwig_wdl_init(_p(A), _p(W), _p(Yhat), _p(Y), _p(C), N, M, int(S), *args,
+        check(lib().rwig_wdl_init(_p(A), _p(W), Yp, _p(Y), _p(C), N, M, int(S), *args,
                                   _p(Alpha0), _p(Lambda0), prec))
+    if sinkhorn_mode == 2:
+        ctrl = {"reg": reg, "method": "log", "max_iter": max_iter, "zero_tol": zero_tol}
+        for m in range(M):
+            Yhat[:, m] = barycenter(A, C, W[:, m], barycenter_control=ctrl)["b"]
     return {"A": A, "W": W, "Yhat": Yhat}

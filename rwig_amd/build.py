@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librwig_hip.so")
-SOURCES = ["runtime.hip", "plain.hip", "batched.hip", "batched_fast.hip", "gemm.hip", "batched_dense.hip", "wdl.hip", "barycenter_ck.hip", "serve.hip", "softmin.hip", "prep.hip"]
+SOURCES = ["runtime.hip", "plain.hip", "batched.hip", "batched_fast.hip", "gemm.hip", "batched_dense.hip", "wdl.hip", "barycenter_ck.hip", "serve.hip", "softmin.hip", "prep.hip", "wig.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -34,7 +34,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Sinkhorn/barycenter solves/s at 1e-6 marginal err vs plain FP64; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TF = 2500.0
-ABSORB_MFMA = 30  # iterations between FP64 absorptions on the MFMA path
+ABSORB_MFMA = 20  # iterations between FP64 absorptions on the MFMA path (the workload's)
+ABSORB_ALT = 30   # a second interval, reported beside the headline
 GEMM_KSPLIT = 1   # k split of the MFMA path's BF16 products (deferred epilogue)
 
 
@@ -211,7 +212,7 @@ def c4_leg(torch, dev, stream, log):
     return out
 
 
-def c5_leg(log, paced_s=8.0):
+def c5_leg(log, paced_s=8.0, graphs=True):
     """Serving mix: 32 cost matrices (n log-uniform in [64, 8192]) resident,
     70% Sinkhorn / 30% barycenter (S in {2, 4, 8}), eps in {0.1, 0.01}, stop at
     L1 marginal error 1e-6. Saturated throughput (1024 requests at once), then
@@ -229,7 +230,8 @@ def c5_leg(log, paced_s=8.0):
     out = {"workload": "serving mix: 32 cost matrices n in [64, 8192] (log-uniform), 70% "
                        "Sinkhorn / 30% barycenter S in {2,4,8}, eps in {0.1, 0.01}, "
                        "L1 marginal err <= 1e-6, BF16 MFMA + FP64 absorption every 20"}
-    with ServeSession(hbm_budget_bytes=64 << 30, max_group_columns=512, absorb_every=20) as s:
+    with ServeSession(hbm_budget_bytes=64 << 30, max_group_columns=512, absorb_every=20,
+                      graphs=graphs) as s:
         ids = [s.register_cost(C) for C in clouds]
         warm = [s.submit_sinkhorn(ids[k], np.full(n, 1.0 / n), np.full(n, 1.0 / n), eps)
                 for k, n in enumerate(sizes) for eps in (0.1, 0.01)]
@@ -286,13 +288,28 @@ def main():
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant product (5 launches) and a 256 MiB copy (5 "
                          "launches): the workload of the HBM-traffic counter runs")
+    ap.add_argument("--only", choices=("c3", "lse", "c4", "c5"),
+                    help="run one secondary measurement alone and print its JSON object "
+                         "(the workload of a profiler run)")
+    ap.add_argument("--serve-eager", action="store_true",
+                    help="serving launches its kernels one by one instead of replaying "
+                         "recorded HIP graphs (same kernels and results; for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched "
-              "solves shard by target with no exchange, see DESIGN.md", file=sys.stderr)
+              "solves shard by target with no exchange between GPUs", file=sys.stderr)
         sys.exit(2)
     if args.gemm_only:
         gemm_only()
+        return
+    if args.only:
+        import torch
+        dev = torch.device("cuda:0")
+        stream = torch.cuda.current_stream()
+        legs = {"c3": lambda: c3_leg(log), "lse": lambda: lse_leg(torch, dev, stream, log),
+                "c4": lambda: c4_leg(torch, dev, stream, log),
+                "c5": lambda: c5_leg(log, graphs=not args.serve_eager)}
+        print(json.dumps(legs[args.only]()), flush=True)
         return
 
     import torch
@@ -331,9 +348,9 @@ def main():
     log("building the Gibbs kernels (BF16 and FP64, both orientations)")
     kern = RB.DenseKernel(Cd, reg)
 
-    def run_mfma(out=None):
+    def run_mfma(out=None, absorb=ABSORB_MFMA):
         return RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, zero_tol=tol,
-                                        max_iter=1000, absorb_every=ABSORB_MFMA, out=out)
+                                        max_iter=1000, absorb_every=absorb, out=out)
 
     out = run_mfma()
     for _ in range(args.warmup):
@@ -351,6 +368,17 @@ def main():
     step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
     ok_mfma, it_mfma, _ = check_out(out, "bf16-mfma")
     value = B * args.steps / wall
+
+    # the same path with absorptions every ABSORB_ALT iterations (fewer FP64
+    # GEMMs, more iterations past the stopping point)
+    out_alt = run_mfma(absorb=ABSORB_ALT)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        run_mfma(out_alt, absorb=ABSORB_ALT)
+    torch.cuda.synchronize()
+    alt_sps = 2 * B / (time.perf_counter() - t0)
+    ok_alt, it_alt, _ = check_out(out_alt, f"bf16-mfma absorb every {ABSORB_ALT}")
 
     # dominant kernel: the BF16 MFMA GEMM K^T W (M x B x M); its average launch
     # time measured here with HIP events on the same stream, same shape
@@ -461,13 +489,15 @@ def main():
             "precision": "BF16 MFMA products (FP32 accumulation), FP64 absorption every "
                          f"{ABSORB_MFMA} iterations (FP64 MFMA marginals)",
         },
-        "converged": {"bf16_mfma": ok_mfma, "bf16_resident": okr, "fp32_resident": ok32,
-                      "fp64_batched": ok64},
+        "converged": {"bf16_mfma": ok_mfma, f"bf16_mfma_absorb{ABSORB_ALT}": ok_alt,
+                      "bf16_resident": okr, "fp32_resident": ok32, "fp64_batched": ok64},
         "iterations_mean": {"bf16_mfma": float(it_mfma.mean()),
+                            f"bf16_mfma_absorb{ABSORB_ALT}": float(it_alt.mean()),
                             "bf16_resident": float(itr.mean()),
                             "fp32_resident": float(it32.mean()),
                             "fp64_batched": float(it64.mean())},
-        "modes_solves_per_s": {"bf16_mfma": value, "bf16_resident": bf16r_sps,
+        "modes_solves_per_s": {"bf16_mfma": value, f"bf16_mfma_absorb{ABSORB_ALT}": alt_sps,
+                               "bf16_resident": bf16r_sps,
                                "fp32_resident": fp32_sps, "fp64_batched": fp64_sps,
                                "fp64_plain_per_call": plain_sps},
         "speedup_vs_plain_fp64": value / plain_sps,

@@ -85,10 +85,15 @@ template <> struct Mfma<float> {
 // (BM/WGM) x (BN/WGN) block of 16 x 16 MFMA tiles, PF k-steps of global loads
 // in flight (PF = 2: a second register set, so that a load has two k-steps of
 // MFMA work to arrive in).
-template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1>
+// SCHED: 1 raises the wavefront's issue priority over its MFMA work (the
+// loads and LDS stores of the other wavefronts then fill its gaps), 2 asks the
+// scheduler to interleave the LDS reads of a k-step with its MFMAs, 3 both.
+template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1,
+          int SCHED_ = 0>
 struct GemmShape {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WGM = WGM_, WGN = WGN_;
   static constexpr int PF = PF_;  // k-steps of global loads in flight (1 or 2)
+  static constexpr int SCHED = SCHED_;
   static constexpr int NT = 64 * WGM * WGN;  // threads per workgroup
   static constexpr int WM = BM / WGM, WN = BN / WGN;
   static constexpr int TM = WM / 16, TN = WN / 16;
@@ -285,6 +290,7 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
   // one k-step of MFMA work on an LDS stage
   auto mma_step = [&](const T *as, const T *bs) {
+    if constexpr (CF::SCHED & 1) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += MF::KI) {
       typename MF::frag_t af[TM], bfr[TN];
@@ -297,6 +303,15 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 #pragma unroll
         for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
     }
+    if constexpr (CF::SCHED & 2) {
+      // one LDS read, then one MFMA, in turn, for the whole k-step
+#pragma unroll
+      for (int q = 0; q < (BK / MF::KI) * (TM + TN); ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      }
+    }
+    if constexpr (CF::SCHED & 1) __builtin_amdgcn_s_setprio(0);
   };
   // global loads of the k-step at kk0 into a register set: one uniform branch
   // around both loaders, so that the full-step loads form one block with no

@@ -184,33 +184,43 @@ void gemv_t(hipStream_t st, const double *K, int M, int N, long ldk,
 // m - reg log t is the two-pass form of src/sinkhorn_impl_log.cpp:48-50 up to
 // rounding.
 
-// exp of the soft-min passes: FP64 keeps the library exp (the FP64 forms match
-// the oracle to 1e-10); FP32 uses the hardware exponential, v_exp_f32 on
-// x log2(e) (about 1 ulp, plus |x| ulp from the product: the arguments here
-// are <= 0 and terms below e^-87 vanish either way), a quarter of the
-// instructions of the range-reduced library form.
-__device__ __forceinline__ double sm_exp(double x) { return exp(x); }
-__device__ __forceinline__ float sm_exp(float x) { return __expf(x); }
+// The passes work in base-2 units of reg: every operand of R is multiplied by
+// sc = log2(e) / reg once, where it is loaded (an entry of C once for all S
+// slices, F, G and the shifts once per row or column), so that
+// exp(-r / reg) = exp2(-r * sc) needs no multiply per (entry, slice). A
+// result m2 - log2(t) converts back with 1 / sc = reg ln 2.
+// exp2: FP64 keeps the library exp2 (the FP64 forms match the oracle to
+// 1e-10); FP32 uses the hardware v_exp_f32 (about 1 ulp; the arguments are
+// <= 0 and terms below 2^-126 vanish either way).
+constexpr double kLog2e = 1.4426950408889634074;
+constexpr double kLn2 = 0.69314718055994530942;
+__device__ __forceinline__ double sm_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ float sm_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
+// Online log-sum-exp state in base-2 units: running minimum m and
+// t = sum exp2(-(r - m)), rescaled when the minimum drops.
 template <typename T> struct SoftMin {
   T m, t;
   __device__ void init() {
     m = T(INFINITY);
     t = T(0);
   }
-  __device__ void add(T r, T inv_reg) {
-    const T d = (r - m) * inv_reg;
-    const T e = sm_exp(-fabs(d));
-    t = (d < T(0)) ? fma(t, e, T(1)) : t + e;
-    m = (r < m) ? r : m;
+  __device__ void add(T r) {
+    const T d = r - m;
+    const T e = sm_exp2(-fabs(d));
+    const bool lt = d < T(0);
+    t = lt ? fma(t, e, T(1)) : t + e;
+    m = lt ? r : m;
   }
-  __device__ void merge(T m2, T t2, T inv_reg) {
+  __device__ void merge(T m2, T t2) {
     const T mn = fmin(m, m2);
-    const T a = (m == mn) ? t : t * sm_exp(-(m - mn) * inv_reg);
-    const T b = (m2 == mn) ? t2 : t2 * sm_exp(-(m2 - mn) * inv_reg);
+    const T a = (m == mn) ? t : t * sm_exp2(mn - m);
+    const T b = (m2 == mn) ? t2 : t2 * sm_exp2(mn - m2);
     m = mn;
     t = a + b;
   }
+  // the soft-min in the caller's units: (m - log2 t) / sc
+  __device__ T value(T back) const { return (m - log2(t)) * back; }
 };
 
 template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int o) {
@@ -233,7 +243,7 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
   if (ls && !ls->active) return;
   __shared__ T shm[kWavesPerBlock][CB * SM], sht[kWavesPerBlock][CB * SM];
   const int j0 = blockIdx.x * CB;
-  const T inv = T(1) / reg;
+  const T sc = T(kLog2e / (double)reg), back = T((double)reg * kLn2);
   T g[CB][SM];
   SoftMin<T> st[CB][SM];
 #pragma unroll
@@ -241,19 +251,19 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
       st[c][s].init();
-      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] : T(0);
+      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] * sc : T(0);
     }
   for (int i = threadIdx.x; i < M; i += kTB) {
     T f[SM];
 #pragma unroll
-    for (int s = 0; s < SM; ++s) f[s] = (s < S) ? F[i + (long)s * ldf] : T(0);
+    for (int s = 0; s < SM; ++s) f[s] = (s < S) ? F[i + (long)s * ldf] * sc : T(0);
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       if (j0 + c >= N) continue;
-      const T x = C[i + (long)(j0 + c) * ldc];
+      const T x = C[i + (long)(j0 + c) * ldc] * sc;
 #pragma unroll
       for (int s = 0; s < SM; ++s)
-        if (s < S) st[c][s].add((x - f[s]) - g[c][s], inv);
+        if (s < S) st[c][s].add((x - f[s]) - g[c][s]);
     }
   }
   // wavefront, then workgroup merge
@@ -265,7 +275,7 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
       if (s >= S) continue;
 #pragma unroll
       for (int o = kWave / 2; o > 0; o >>= 1)
-        st[c][s].merge(shfl_xor_t(st[c][s].m, o), shfl_xor_t(st[c][s].t, o), inv);
+        st[c][s].merge(shfl_xor_t(st[c][s].m, o), shfl_xor_t(st[c][s].t, o));
       if (lane == 0) {
         shm[w][c * SM + s] = st[c][s].m;
         sht[w][c * SM + s] = st[c][s].t;
@@ -282,8 +292,8 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
       SoftMin<T> a;
       a.m = shm[0][c * SM + s];
       a.t = sht[0][c * SM + s];
-      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][c * SM + s], sht[v][c * SM + s], inv);
-      res[s] = a.m - reg * log(a.t);
+      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][c * SM + s], sht[v][c * SM + s]);
+      res[s] = a.value(back);
     }
     op(j0 + c, res);
   }
@@ -332,12 +342,20 @@ inline Chunking row_chunking(int M, int N) {
 constexpr int kColTile = 64;
 constexpr int kColUnroll = 4;  // columns per wavefront whose loads are in flight together
 
+// per-column values of columns t0 .. t0 + nc into LDS as [column][slice],
+// times sc; with src2, (src + src2) * sc
 template <typename T, int SM>
 __device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, int t0, int nc,
-                                           int S, T *dst) {
+                                           int S, T *dst, T sc,
+                                           const T *__restrict__ src2 = nullptr, long ld2 = 0) {
   for (int k = threadIdx.x; k < nc * SM; k += kTB) {
     const int jj = k / SM, s = k % SM;
-    dst[k] = (s < S) ? src[(t0 + jj) + (long)s * ld] : T(0);
+    T v = T(0);
+    if (s < S) {
+      v = src[(t0 + jj) + (long)s * ld];
+      if (src2) v += src2[(t0 + jj) + (long)s * ld2];
+    }
+    dst[k] = v * sc;
   }
 }
 
@@ -355,20 +373,20 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
   const int i = blockIdx.x * kRowsPerBlock + lane;
   const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const T inv = T(1) / reg;
+  const T sc = T(kLog2e / (double)reg);
   const bool row = i < M;
   T f[SM];
   SoftMin<T> st[SM];
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
     st[s].init();
-    f[s] = (row && s < S) ? F[i + (long)s * ldf] : T(0);
+    f[s] = (row && s < S) ? F[i + (long)s * ldf] * sc : T(0);
   }
   __shared__ T gs[kColTile * SM];
   for (int t0 = j0; t0 < j1; t0 += kColTile) {
     const int nc = min(kColTile, j1 - t0);
     __syncthreads();
-    stage_cols<T, SM>(G, ldg, t0, nc, S, gs);
+    stage_cols<T, SM>(G, ldg, t0, nc, S, gs, sc);
     __syncthreads();
     if (row) {
       // kColUnroll columns' loads issued ahead of their updates
@@ -381,15 +399,16 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
 #pragma unroll
         for (int u = 0; u < kColUnroll; ++u) {
           const T *g = gs + (jj + u * kWavesPerBlock) * SM;
+          const T xs = x[u] * sc;
 #pragma unroll
-          for (int s = 0; s < SM; ++s) st[s].add((x[u] - f[s]) - g[s], inv);
+          for (int s = 0; s < SM; ++s) st[s].add((xs - f[s]) - g[s]);
         }
       }
       for (; jj < nc; jj += kWavesPerBlock) {
-        const T x = C[i + (long)(t0 + jj) * ldc];
+        const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
         const T *g = gs + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s) st[s].add((x - f[s]) - g[s], inv);
+        for (int s = 0; s < SM; ++s) st[s].add((xs - f[s]) - g[s]);
       }
     }
   }
@@ -405,7 +424,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
       SoftMin<T> a;
       a.m = shm[0][lane];
       a.t = sht[0][lane];
-      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane], inv);
+      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane]);
       const long o = ((long)c * S + s) * M + i;
       pm[o] = a.m;
       ps[o] = a.t;
@@ -418,14 +437,14 @@ __global__ void __launch_bounds__(kTB)
 k_lse_row_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, long len, T reg,
                 Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  const T inv = T(1) / reg;
+  const T back = T((double)reg * kLn2);
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
        k += (long)gridDim.x * blockDim.x) {
     SoftMin<T> a;
     a.m = pm[k];
     a.t = ps[k];
-    for (int c = 1; c < nchunk; ++c) a.merge(pm[(long)c * len + k], ps[(long)c * len + k], inv);
-    op(k, a.m - reg * log(a.t));
+    for (int c = 1; c < nchunk; ++c) a.merge(pm[(long)c * len + k], ps[(long)c * len + k]);
+    op(k, a.value(back));
   }
 }
 
@@ -465,30 +484,29 @@ k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
   if (ls && !ls->active) return;
   __shared__ T sha[kWavesPerBlock][CB * SM];
   const int j0 = blockIdx.x * CB;
-  const T inv = T(1) / reg;
+  const T sc = T(kLog2e / (double)reg);
   T g[CB][SM], acc[CB][SM];
 #pragma unroll
   for (int c = 0; c < CB; ++c)
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
       acc[c][s] = T(0);
-      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] : T(0);
+      g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] * sc : T(0);
     }
   for (int i = threadIdx.x; i < M; i += kTB) {
-    T f[SM], sh[SM], xv[SM];
+    T fs[SM], xv[SM];  // fs = (F + rshift) * sc: the row's two shifts in one
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
-      f[s] = (s < S) ? F[i + (long)s * ldf] : T(0);
-      sh[s] = (s < S) ? rshift[i + (long)s * M] : T(0);
+      fs[s] = (s < S) ? (F[i + (long)s * ldf] + rshift[i + (long)s * M]) * sc : T(0);
       xv[s] = (s < S) ? x[i + (long)s * M] : T(0);
     }
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       if (j0 + c >= N) continue;
-      const T cij = C[i + (long)(j0 + c) * ldc];
+      const T cij = C[i + (long)(j0 + c) * ldc] * sc;
 #pragma unroll
       for (int s = 0; s < SM; ++s)
-        if (s < S) acc[c][s] += sm_exp(-(((cij - f[s]) - g[c][s]) - sh[s]) * inv) * xv[s];
+        if (s < S) acc[c][s] += sm_exp2(-((cij - fs[s]) - g[c][s])) * xv[s];
     }
   }
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -551,7 +569,7 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
   const int i = blockIdx.x * kRowsPerBlock + lane;
   const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
-  const T inv = T(1) / reg;
+  const T sc = T(kLog2e / (double)reg);
   const bool row = i < M;
   T f[SM], acc[SM];
 #pragma unroll
@@ -561,15 +579,16 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
     if (row && s < S) {
       f[s] = F[i + (long)s * ldf];
       if (rowsh) f[s] -= rowsh[i + (long)s * M];
+      f[s] *= sc;
     }
   }
-  __shared__ T gs[kColTile * SM], hs[kColTile * SM], ys[kColTile * SM];
+  // gh = (G + cshift) * sc: the column's two shifts in one
+  __shared__ T gh[kColTile * SM], ys[kColTile * SM];
   for (int t0 = j0; t0 < j1; t0 += kColTile) {
     const int nc = min(kColTile, j1 - t0);
     __syncthreads();
-    stage_cols<T, SM>(G, ldg, t0, nc, S, gs);
-    stage_cols<T, SM>(cshift, N, t0, nc, S, hs);
-    stage_cols<T, SM>(y, N, t0, nc, S, ys);
+    stage_cols<T, SM>(G, ldg, t0, nc, S, gh, sc, cshift, N);
+    stage_cols<T, SM>(y, N, t0, nc, S, ys, T(1));
     __syncthreads();
     if (row) {
       int jj = w;
@@ -581,16 +600,16 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
 #pragma unroll
         for (int u = 0; u < kColUnroll; ++u) {
           const int q = (jj + u * kWavesPerBlock) * SM;
+          const T xs = x[u] * sc;
 #pragma unroll
-          for (int s = 0; s < SM; ++s)
-            acc[s] += sm_exp(-(((x[u] - f[s]) - gs[q + s]) - hs[q + s]) * inv) * ys[q + s];
+          for (int s = 0; s < SM; ++s) acc[s] += sm_exp2(-((xs - f[s]) - gh[q + s])) * ys[q + s];
         }
       }
       for (; jj < nc; jj += kWavesPerBlock) {
-        const T x = C[i + (long)(t0 + jj) * ldc];
-        const T *g = gs + jj * SM, *h = hs + jj * SM, *yv = ys + jj * SM;
+        const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
+        const T *g = gh + jj * SM, *yv = ys + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s) acc[s] += sm_exp(-(((x - f[s]) - g[s]) - h[s]) * inv) * yv[s];
+        for (int s = 0; s < SM; ++s) acc[s] += sm_exp2(-((xs - f[s]) - g[s])) * yv[s];
       }
     }
   }
@@ -677,7 +696,7 @@ k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, 
   if (ls && !ls->active) return;
   constexpr int JB = kTB / SM;
   __shared__ T r[JB][SM];
-  const T inv = T(1) / reg;
+  const T back = T((double)reg * kLn2);
   const int jj = threadIdx.x / SM, s = threadIdx.x % SM;
   const int j = blockIdx.x * JB + jj;
   if (j < N && s < S) {
@@ -685,8 +704,8 @@ k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, 
     const long k = (long)s * N + j, len = (long)S * N;
     a.m = pm[k];
     a.t = ps[k];
-    for (int c = 1; c < nchunk; ++c) a.merge(pm[c * len + k], ps[c * len + k], inv);
-    r[jj][s] = a.m - reg * log(a.t);
+    for (int c = 1; c < nchunk; ++c) a.merge(pm[c * len + k], ps[c * len + k]);
+    r[jj][s] = a.value(back);
   }
   __syncthreads();
   if (threadIdx.x < JB) {

@@ -34,6 +34,10 @@ namespace rwig {
 
 namespace {
 
+// tiles of the BF16 products once at most kNarrowTargets targets are active
+using NarrowBf16 = GemmShape<uint16_t, 64, 64, 64, 16, 2, 2, 2, 1>;
+constexpr int kNarrowTargets = 256;
+
 // per-target L1 (or reference) marginal error of the absorbed plan, stopping
 // decision, form of the next cycle; one block per target
 __global__ void __launch_bounds__(256)
@@ -180,6 +184,17 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   const uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
   const double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
   int nact = B;
+  // BF16 products over the active targets: 128 x 128 tiles, or 64 x 64 tiles
+  // once few targets remain (four times the workgroups for the same columns).
+  // Each output element sees the same sequence of MFMAs either way, so a
+  // target's arithmetic does not depend on which tiles its columns fall in.
+  auto bf16_product = [&](int rows, int kdim, const uint16_t *Am, long la, const uint16_t *Bm,
+                          long lb, auto op) {
+    if (nact <= kNarrowTargets)
+      gemm_tn<uint16_t, NarrowBf16>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
+    else
+      gemm_tn<uint16_t>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
+  };
 
   // ---- initial state
   launch_map(st, MB, [=] __device__(long k) {
@@ -258,16 +273,12 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     launch_map(st, (long)nnz, [=] __device__(long p) {
       X[col_idx[p] + (long)tgt_of[p] * ldn] = f2bf(be[p]);
     });
-    gemm_tn<uint16_t>(
-        st, M, nact, Nc, KbT, ldn, X, ldn,
-        [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; }, act, nullptr, skb);
-    gemm_tn<uint16_t>(
-        st, Nc, nact, M, Kb, ldm, W, ldm,
-        [=] __device__(int j, int t, float z) {
-          const int p = pos[j + (long)t * Nc];
-          if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
-        },
-        act, nullptr, skb);
+    bf16_product(M, Nc, KbT, ldn, X, ldn,
+                 [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; });
+    bf16_product(Nc, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
+      const int p = pos[j + (long)t * Nc];
+      if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
+    });
     // u = a / r0 with v = 1; W for the first product of the cycle
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
@@ -285,7 +296,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
 
   auto iterate = [&]() {
     // Z = K^T W -> v, X
-    gemm_tn<uint16_t>(st, Nc, nact, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
+    bf16_product(Nc, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
       const int p = pos[j + (long)t * Nc];
       if (p < 0 || done[t]) return;
       double v;
@@ -297,9 +308,9 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
         X[j + (long)t * ldn] = f2bf((float)((double)be[p] * v));
       }
       lv[p] = mode[t] ? (float)(v - 1.0) : (float)v;
-    }, act, nullptr, skb);
+    });
     // Y = K X -> u, W
-    gemm_tn<uint16_t>(st, M, nact, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
+    bf16_product(M, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
       if (done[t]) return;
       const long k = i + (long)t * M;
       double u;
@@ -311,7 +322,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
         W[i + (long)t * ldm] = f2bf((float)((double)al[k] * u));
       }
       lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
-    }, act, nullptr, skb);
+    });
   };
 
   // active list: every target at first; after each absorption the targets

@@ -85,38 +85,47 @@ template <> struct Mfma<float> {
 // (BM/WGM) x (BN/WGN) block of 16 x 16 MFMA tiles, PF k-steps of global loads
 // in flight (PF = 2: a second register set, so that a load has two k-steps of
 // MFMA work to arrive in).
-// SCHED: 1 raises the wavefront's issue priority over its MFMA work (the
-// loads and LDS stores of the other wavefronts then fill its gaps), 2 asks the
-// scheduler to interleave the LDS reads of a k-step with its MFMAs, 3 both.
+// DMA = 1: the tiles go from global memory straight into LDS
+// (global_load_lds_dwordx4, no register staging) through PF + 1 LDS stages, PF
+// k-steps ahead, in an unpadded layout whose 16-byte chunks are XOR-swizzled
+// by row (PAD is then unused). Needs 16-bit elements, K a multiple of BK and
+// 16-byte aligned rows; gemm_tn falls back to the register path otherwise.
 template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1,
-          int SCHED_ = 0>
+          int DMA_ = 0>
 struct GemmShape {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WGM = WGM_, WGN = WGN_;
-  static constexpr int PF = PF_;  // k-steps of global loads in flight (1 or 2)
-  static constexpr int SCHED = SCHED_;
+  static constexpr int PF = PF_;  // k-steps of global loads in flight (1 to 4)
+  static constexpr int DMA = DMA_;
   static constexpr int NT = 64 * WGM * WGN;  // threads per workgroup
   static constexpr int WM = BM / WGM, WN = BN / WGN;
   static constexpr int TM = WM / 16, TN = WN / 16;
   static constexpr int LDL = BK + PAD;
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave block must be a multiple of 16");
   static_assert(BK % Mfma<T>::KI == 0, "k-step must be a multiple of the MFMA k");
+  static_assert(!DMA || (sizeof(T) == 2 && (BK == 64 || BK == 128) &&
+                         BM % (WGM * WGN * 8 * 64 / BK) == 0 &&
+                         BN % (WGM * WGN * 8 * 64 / BK) == 0),
+                "the direct-to-LDS path takes 16-bit tiles of 64 or 128 k (8 or 16 chunks of "
+                "16 bytes a row) whose rows split into 1 KB moves per wavefront");
 };
 
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : GemmShape<float, 128, 128, 16, 4, 2, 2> {};
-template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 2, 2> {};
+template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 2, 2, 1> {};
 template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
 
 // Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix
 // stored with its k values contiguous (element (r, k) at p[r * ld + k]) into
-// registers; rows >= nrows and k >= K read as zero. Vector width: 16 bytes.
+// a register set; rows >= nrows and k >= K read as zero. Vector width: 16
+// bytes. The row pointers are shared by all the register sets of a product
+// (a set per k-step of loads in flight).
 template <typename T, int ROWS, int BK, int NT>
 struct TileLoader {
   static constexpr int VEC = 16 / sizeof(T);
   static constexpr int CHUNKS = ROWS * BK / VEC;      // 16-byte chunks per tile
   static constexpr int PER_THREAD = (CHUNKS + NT - 1) / NT;
   static constexpr bool EVEN = CHUNKS % NT == 0;      // every thread has PER_THREAD chunks
-  uint4 reg[PER_THREAD];
+  using Regs = uint4[PER_THREAD];
   const T *row_ptr[PER_THREAD];  // start of this thread's rows (a valid row if outside)
   unsigned inside = 0;           // bit c: chunk c lies on a row of the matrix
 
@@ -139,7 +148,7 @@ struct TileLoader {
     }
   }
   // loads k values k0 .. k0 + BK of the tile's rows; needs k0 + BK <= K
-  __device__ void load_full(int k0) {
+  __device__ void load_full(Regs &reg, int k0) const {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
@@ -149,7 +158,7 @@ struct TileLoader {
     }
   }
   // the last, partial k-step: k >= K read as zero
-  __device__ void load_tail(int K, int k0) {
+  __device__ void load_tail(Regs &reg, int K, int k0) const {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
@@ -169,15 +178,9 @@ struct TileLoader {
       reg[c] = v;
     }
   }
-  __device__ void load(int K, int k0) {
-    if (k0 + BK <= K)
-      load_full(k0);
-    else
-      load_tail(K, k0);
-  }
   // the loaded chunks into the LDS tile; chunks outside the matrix as zero
   // (applied here, after the k-step's MFMA work, not at the load)
-  __device__ void store(T *lds, int ldl) const {
+  __device__ void store(const Regs &reg, T *lds, int ldl) const {
 #pragma unroll
     for (int c = 0; c < PER_THREAD; ++c) {
       const int idx = threadIdx.x + c * NT;
@@ -250,16 +253,29 @@ __device__ __forceinline__ void gemm_epilogue(
     }
 }
 
-template <typename T, class CF, class Op>
-__global__ void __launch_bounds__(CF::NT)
-k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
-          long ldb, const int *__restrict__ nmap, const int *__restrict__ ncount, SplitK<T> sk,
-          Op op) {
-  if (ncount) {  // column count read on the device (the grid covers an upper bound)
-    const int nc = *ncount;
-    if ((int)blockIdx.y * CF::BN >= nc) return;
-    N = nc;
-  }
+#define RWIG_LDS __attribute__((address_space(3)))
+
+// 16 bytes per lane from global memory into LDS at l + 16 * lane (l uniform
+// across the wavefront): global_load_lds_dwordx4. A plain device function, so
+// that the host pass of a template that calls it never sees the builtin.
+__device__ __forceinline__ void lds_move16(const void *g, RWIG_LDS void *l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(v) with the other counters left alone (gfx9 encoding: vmcnt
+// in bits 3:0 and 15:14, expcnt 7 in 6:4, lgkmcnt 15 in 11:8)
+template <int V> __device__ __forceinline__ void wait_vmcnt() {
+  static_assert(V >= 0 && V < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((V & 15) | (7 << 4) | (15 << 8) | ((V >> 4) << 14));
+}
+
+// Main loop, register path: global loads into PF register sets, stored into
+// one of two padded LDS stages after the MFMA work of the current k-step.
+template <typename T, class CF>
+__device__ __forceinline__ void gemm_main_regs(
+    int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb,
+    const int *__restrict__ nmap, int m0, int n0, int kbeg, int kend,
+    typename Mfma<T>::acc_t (&acc)[CF::TM][CF::TN]) {
   using MF = Mfma<T>;
   constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, LDL = CF::LDL;
   constexpr int WM = CF::WM, WN = CF::WN;  // per-wave block
@@ -268,29 +284,17 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
   // being multiplied, one barrier per k-step
   __shared__ __attribute__((aligned(16))) T As[2][BM * LDL];
   __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDL];
-
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wm = wv % CF::WGM, wn = wv / CF::WGM;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-
-  typename MF::acc_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
-
-  TileLoader<T, BM, BK, CF::NT> la;
-  TileLoader<T, BN, BK, CF::NT> lb;
-  // k range of this split (multiples of BK)
-  const int steps = (K + BK - 1) / BK;
-  const int per = (steps + sk.ks - 1) / sk.ks;
-  const int kbeg = blockIdx.z * per * BK, kend = min(K, ((int)blockIdx.z + 1) * per * BK);
+  using LA = TileLoader<T, BM, BK, CF::NT>;
+  using LB = TileLoader<T, BN, BK, CF::NT>;
+  LA la;
+  LB lb;
   la.init(A, lda, M, m0, nullptr);
   lb.init(B, ldb, N, n0, nmap);
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;
   // one k-step of MFMA work on an LDS stage
   auto mma_step = [&](const T *as, const T *bs) {
-    if constexpr (CF::SCHED & 1) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += MF::KI) {
       typename MF::frag_t af[TM], bfr[TN];
@@ -303,72 +307,189 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 #pragma unroll
         for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
     }
-    if constexpr (CF::SCHED & 2) {
-      // one LDS read, then one MFMA, in turn, for the whole k-step
-#pragma unroll
-      for (int q = 0; q < (BK / MF::KI) * (TM + TN); ++q) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      }
-    }
-    if constexpr (CF::SCHED & 1) __builtin_amdgcn_s_setprio(0);
   };
   // global loads of the k-step at kk0 into a register set: one uniform branch
   // around both loaders, so that the full-step loads form one block with no
   // join between them and the compiler defers their waits to the LDS stores
   // that follow the MFMA work
-  auto issue = [&](TileLoader<T, BM, BK, CF::NT> &a, TileLoader<T, BN, BK, CF::NT> &b, int kk0) {
+  auto issue = [&](typename LA::Regs &ra, typename LB::Regs &rb, int kk0) {
     if (kk0 + BK <= K) {
-      a.load_full(kk0);
-      b.load_full(kk0);
+      la.load_full(ra, kk0);
+      lb.load_full(rb, kk0);
     } else {
-      a.load_tail(K, kk0);
-      b.load_tail(K, kk0);
+      la.load_tail(ra, K, kk0);
+      lb.load_tail(rb, K, kk0);
     }
   };
-  issue(la, lb, kbeg);
-  la.store(As[0], LDL);
-  lb.store(Bs[0], LDL);
-  if constexpr (CF::PF == 1) {
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-      const bool more = k0 + BK < kend;
-      if (more) issue(la, lb, k0 + BK);
-      mma_step(As[cur], Bs[cur]);
-      if (more) {
-        la.store(As[cur ^ 1], LDL);
-        lb.store(Bs[cur ^ 1], LDL);
+  // PF register sets: set q holds k-step s with s % PF == q. At step s the
+  // LDS stage s % 2 holds step s, sets (s+1 .. s+PF-1) % PF have their loads
+  // in flight, and set s % PF (stored to LDS at step s - 1) takes step s + PF;
+  // after the MFMA work, step s + 1 goes from its set to the other stage. One
+  // barrier per k-step.
+  constexpr int PF = CF::PF;
+  typename LA::Regs ra[PF];
+  typename LB::Regs rb[PF];
+  const int nst = (kend - kbeg + BK - 1) / BK;
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+    if (q < nst) issue(ra[q], rb[q], kbeg + q * BK);
+  if (nst > 0) {
+    la.store(ra[0], As[0], LDL);
+    lb.store(rb[0], Bs[0], LDL);
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < nst; s0 += PF) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int s = s0 + q;
+      if (s < nst) {
+        if (s + PF < nst) issue(ra[q], rb[q], kbeg + (s + PF) * BK);
+        mma_step(As[s & 1], Bs[s & 1]);
+        if (s + 1 < nst) {
+          la.store(ra[(q + 1) % PF], As[(s + 1) & 1], LDL);
+          lb.store(rb[(q + 1) % PF], Bs[(s + 1) & 1], LDL);
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      cur ^= 1;
-    }
-  } else {
-    // two register sets: at step s, LDS stage s % 2 holds step s, the other
-    // set's loads of step s + 1 are in flight, and this set takes step s + 2
-    TileLoader<T, BM, BK, CF::NT> la2 = la;
-    TileLoader<T, BN, BK, CF::NT> lb2 = lb;
-    const int nst = (kend - kbeg + BK - 1) / BK;
-    if (nst > 1) issue(la2, lb2, kbeg + BK);
-    __syncthreads();
-    for (int s = 0; s < nst; s += 2) {
-      if (s + 2 < nst) issue(la, lb, kbeg + (s + 2) * BK);
-      mma_step(As[0], Bs[0]);
-      if (s + 1 < nst) {
-        la2.store(As[1], LDL);
-        lb2.store(Bs[1], LDL);
-      }
-      __syncthreads();
-      if (s + 1 >= nst) break;
-      if (s + 3 < nst) issue(la2, lb2, kbeg + (s + 3) * BK);
-      mma_step(As[1], Bs[1]);
-      if (s + 2 < nst) {
-        la.store(As[0], LDL);
-        lb.store(Bs[0], LDL);
-      }
-      __syncthreads();
     }
   }
+}
+
+// Main loop, direct-to-LDS path (CF::DMA): each global_load_lds_dwordx4
+// moves 1 KB (64 lanes x 16 bytes, written to LDS in lane order): 8 tile rows
+// of 64 k or 4 of 128 k. The lane with slot s of tile row r loads the 16-byte
+// k chunk s ^ (r % 8), so that chunk c of row r sits at slot c ^ (r % 8) and
+// the MFMA fragment reads (16 rows, one chunk) spread over all LDS banks.
+// Rows past the matrix read a valid row instead: they only feed outputs the
+// epilogue discards. PF + 1 stages; the barrier of step s follows this
+// wavefront's wait for its own moves of step s (the moves of later steps stay
+// in flight), and then step s + PF reuses the stage of step s - 1.
+template <typename T, class CF, class Acc>
+__device__ __forceinline__ void gemm_main_dma(
+    int M, int N, const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb,
+    const int *__restrict__ nmap, int m0, int n0, int kbeg, int kend, Acc &acc) {
+  using MF = Mfma<T>;
+  constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK;
+  constexpr int WM = CF::WM, WN = CF::WN, TM = CF::TM, TN = CF::TN;
+  constexpr int NW = CF::WGM * CF::WGN;
+  constexpr int ROWB = BK * (int)sizeof(T);  // bytes of a tile row: 8 or 16 chunks
+  constexpr int CPR = ROWB / 16;             // chunks per row
+  constexpr int RPM = 64 / CPR;              // rows per move (64 lanes x 16 bytes)
+  constexpr int PF = CF::PF, STAGES = PF + 1;
+  constexpr int STAGEB = (BM + BN) * ROWB;
+  constexpr int MA = BM / NW / RPM, MB = BN / NW / RPM;  // moves per wavefront and k-step
+  constexpr int MOVES = MA + MB;
+  __shared__ __attribute__((aligned(1024))) char smem[STAGES * STAGEB];
+  RWIG_LDS char *lds = (RWIG_LDS char *)smem;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv % CF::WGM, wn = wv / CF::WGM;
+  // the lane's row within the move and its LDS slot; the chunk it loads is
+  // slot ^ (tile row % 8)
+  const int sub = lane / CPR, slot = lane % CPR;
+  const T *pa[MA], *pb[MB];
+#pragma unroll
+  for (int q = 0; q < MA; ++q) {
+    const int rl = (wv * MA + q) * RPM + sub, r = m0 + rl;
+    pa[q] = A + (long)(r < M ? r : 0) * lda + (slot ^ (rl & 7)) * 8;
+  }
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int rl = (wv * MB + q) * RPM + sub, n = n0 + rl;
+    const long r = n < N ? (nmap ? nmap[n] : n) : (nmap ? nmap[0] : 0);
+    pb[q] = B + r * ldb + (slot ^ (rl & 7)) * 8;
+  }
+  const int nst = (kend - kbeg) / BK;
+  auto issue = [&](int s) {
+    RWIG_LDS char *st = lds + (s % STAGES) * STAGEB;
+    const int k = kbeg + s * BK;
+#pragma unroll
+    for (int q = 0; q < MA; ++q)
+      lds_move16(pa[q] + k, (RWIG_LDS void *)(st + (wv * MA + q) * RPM * ROWB));
+#pragma unroll
+    for (int q = 0; q < MB; ++q)
+      lds_move16(pb[q] + k, (RWIG_LDS void *)(st + (BM + (wv * MB + q) * RPM) * ROWB));
+  };
+#pragma unroll
+  for (int q = 0; q < PF; ++q)
+    if (q < nst) issue(q);
+  const int lr = lane & 15, lc = (lane >> 4) * MF::KL / 8;  // the lane's chunk within a k-block
+  auto frag = [&](const RWIG_LDS char *tile, int row, int c) -> typename MF::frag_t {
+    return *reinterpret_cast<const RWIG_LDS typename MF::frag_t *>(tile + row * ROWB +
+                                                                   ((c ^ (row & 7)) << 4));
+  };
+  for (int s = 0; s < nst; ++s) {
+    // this wavefront's moves of step s have landed once only the later
+    // steps' moves are outstanding
+    const int later = min(nst, s + PF) - (s + 1);
+    if constexpr (PF >= 4) {
+      if (later >= 3) wait_vmcnt<3 * MOVES>();
+      else if (later == 2) wait_vmcnt<2 * MOVES>();
+      else if (later == 1) wait_vmcnt<MOVES>();
+      else wait_vmcnt<0>();
+    } else if constexpr (PF == 3) {
+      if (later >= 2) wait_vmcnt<2 * MOVES>();
+      else if (later == 1) wait_vmcnt<MOVES>();
+      else wait_vmcnt<0>();
+    } else if constexpr (PF == 2) {
+      if (later >= 1) wait_vmcnt<MOVES>();
+      else wait_vmcnt<0>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (s + PF < nst) issue(s + PF);
+    const RWIG_LDS char *as = lds + (s % STAGES) * STAGEB, *bs = as + BM * ROWB;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += MF::KI) {
+      const int c = kk / 8 + lc;
+      typename MF::frag_t af[TM], bfr[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = frag(as, wm * WM + i * 16 + lr, c);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bfr[j] = frag(bs, wn * WN + j * 16 + lr, c);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
+    }
+  }
+}
+
+template <typename T, class CF, class Op>
+__global__ void __launch_bounds__(CF::NT)
+k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__restrict__ B,
+          long ldb, const int *__restrict__ nmap, const int *__restrict__ ncount, SplitK<T> sk,
+          Op op) {
+  if (ncount) {  // column count read on the device (the grid covers an upper bound)
+    const int nc = *ncount;
+    if ((int)blockIdx.y * CF::BN >= nc) return;
+    N = nc;
+  }
+  using MF = Mfma<T>;
+  constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK;
+  constexpr int WM = CF::WM, WN = CF::WN;  // per-wave block
+  constexpr int TM = CF::TM, TN = CF::TN;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wm = wv % CF::WGM, wn = wv / CF::WGM;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+
+  typename MF::acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
+
+  // k range of this split (multiples of BK)
+  const int steps = (K + BK - 1) / BK;
+  const int per = (steps + sk.ks - 1) / sk.ks;
+  const int kbeg = blockIdx.z * per * BK, kend = min(K, ((int)blockIdx.z + 1) * per * BK);
+  if constexpr (CF::DMA)
+    gemm_main_dma<T, CF>(M, N, A, lda, B, ldb, nmap, m0, n0, kbeg, kend, acc);
+  else
+    gemm_main_regs<T, CF>(M, N, K, A, lda, B, ldb, nmap, m0, n0, kbeg, kend, acc);
+  const int lr = lane & 15;
   if (sk.work) {
     // partial tile of this split, column-major within the tile; the sum and
     // the epilogue follow in k_gemm_splitk_epilogue (the kernel boundary
@@ -436,6 +557,19 @@ template <typename T, class CF = GemmCfg<T>, class Op>
 void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
              Op op, const int *nmap = nullptr, const int *ncount = nullptr,
              const SplitK<T> &sk = SplitK<T>()) {
+  if constexpr (CF::DMA) {
+    // the direct-to-LDS path moves whole 16-byte chunks of whole k-steps: other
+    // shapes take the register path with the same tiles (same arithmetic,
+    // same split workspace)
+    const bool ok = K % CF::BK == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+                    (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+    if (!ok) {
+      using Reg = GemmShape<T, CF::BM, CF::BN, CF::BK, 16, CF::WGM, CF::WGN, 2, 0>;
+      gemm_tn<T, Reg>(st, M, N, K, A, lda, B, ldb, op, nmap, ncount, sk);
+      return;
+    }
+  }
   constexpr int BM = CF::BM, BN = CF::BN;
   RWIG_REQUIRE(M > 0 && N >= 0 && K > 0, "GEMM dimensions must be positive");
   RWIG_REQUIRE(lda % (16 / sizeof(T)) == 0 && ldb % (16 / sizeof(T)) == 0,

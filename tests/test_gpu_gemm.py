@@ -33,12 +33,15 @@ def run_bf16(A, Bt, M, N, K, ks=1, deferred=False):
     return D
 
 
-# (ks, deferred): fused epilogue; deferred epilogue; k split in 2 and 5
+# (ks, deferred): fused epilogue; deferred epilogue; k split in 2 and 5.
+# Shapes with K a multiple of 64 take the direct-to-LDS path, the others the
+# register path.
 SPLITS = [(1, False), (1, True), (2, True), (5, True)]
 
 
 @pytest.mark.parametrize("ks,deferred", SPLITS)
-@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (128, 128, 64), (300, 70, 200), (1000, 257, 999)])
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (128, 128, 64), (300, 70, 200), (1000, 257, 999),
+                                   (300, 70, 256), (129, 600, 64), (1000, 257, 1024)])
 def test_bf16_gemm_matches_torch(M, N, K, ks, deferred):
     g = torch.Generator(device="cpu").manual_seed(M + N + K)
     lda = (K + 7) // 8 * 8
@@ -98,3 +101,27 @@ def test_bf16_gemm_throughput_report(ks, deferred):
           f"{2*M*N*K/dt/1e12:.1f} TFLOP/s", flush=True)
     ref = (Bt[:64].float() @ A.float().T)
     assert torch.allclose(D[:64], ref, rtol=1e-2, atol=1e-1)
+
+
+@pytest.mark.parametrize("ks,deferred", [(1, False), (2, True)])
+@pytest.mark.parametrize("M,N,K", [(300, 70, 256), (1000, 257, 1024)])
+def test_bf16_direct_to_lds_equals_register_path(M, N, K, ks, deferred):
+    # the same operands once 16-byte aligned (direct-to-LDS path) and once
+    # shifted by 4 elements (rows no longer 16-byte aligned: register path)
+    g = torch.Generator(device="cpu").manual_seed(7 * M + K)
+    lda = K + 8
+    A = torch.randn((M, lda), generator=g).to(torch.bfloat16).cuda()
+    Bt = torch.randn((N, lda), generator=g).to(torch.bfloat16).cuda()
+    Da = run_bf16(A, Bt, M, N, K, ks, deferred)
+    As = torch.empty((M, lda + 8), dtype=torch.bfloat16, device="cuda")
+    Bs = torch.empty((N, lda + 8), dtype=torch.bfloat16, device="cuda")
+    As[:, 4:4 + lda] = A
+    Bs[:, 4:4 + lda] = Bt
+    D = torch.empty((N, M), dtype=torch.float32, device="cuda")
+    w = workspace(0, M, N, ks) if deferred else None
+    check(lib().rwig_gemm_tn_bf16_dev(M, N, K, As[:, 4:].data_ptr(), lda + 8,
+                                      Bs[:, 4:].data_ptr(), lda + 8, D.data_ptr(), M, ks,
+                                      None if w is None else w.data_ptr(),
+                                      torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert torch.equal(Da, D)

@@ -170,9 +170,9 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   double *Bd = alloc_into<double>(pool, (size_t)ldn * B);
   float *Yc = alloc_into<float>(pool, MB);
   // deferred GEMM epilogues (the fused updates run in their own kernel, with
-  // the occupancy their loads need); no k split: with 8-wave workgroups and two
-  // k-steps of loads in flight, one workgroup per CU hides the load latency
-  // (measured at 1024 targets: 16.0k solves/s, 15.6k with a split in two)
+  // the occupancy their loads need); no k split: at 1024 targets the 128 x 128
+  // tiles give one workgroup per CU already, and a split in two measured
+  // slower (15.6k against 16.0k solves/s with the register-path tiles)
   SplitK<uint16_t> skb;
   skb.ks = 1;
   skb.work = alloc_into<float>(

@@ -111,7 +111,9 @@ struct GemmShape {
 
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : GemmShape<float, 128, 128, 16, 4, 2, 2> {};
-template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 2, 2, 1> {};
+// BF16: 16 wavefronts (4 x 4, 32 x 32 each), 3 k-steps of direct-to-LDS
+// moves in flight (4 LDS stages, 128 KB)
+template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 4, 3, 1> {};
 template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
 
 // Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix

@@ -179,7 +179,9 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       pool, std::max(splitk_work_elems<uint16_t>(M, B, skb.ks), splitk_work_elems<uint16_t>(Nc, B, skb.ks)));
   SplitK<double> skd;
   skd.work = alloc_into<double>(
-      pool, std::max(splitk_work_elems<double>(M, B, 1), splitk_work_elems<double>(Nc, B, 1)));
+      pool, std::max({splitk_work_elems<double>(M, B, 1), splitk_work_elems<double>(Nc, B, 1),
+                      splitk_work_elems<double, GemmF64Wide>(M, B, 1),
+                      splitk_work_elems<double, GemmF64Wide>(Nc, B, 1)}));
 
   const uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
   const double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
@@ -194,6 +196,14 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       gemm_tn<uint16_t, NarrowBf16>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
     else
       gemm_tn<uint16_t>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
+  };
+  // FP64 products: the wide tiles while many targets are active
+  auto f64_product = [&](int rows, int kdim, const double *Am, long la, const double *Bm,
+                         long lb, auto op) {
+    if (nact <= kNarrowTargets)
+      gemm_tn<double>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skd);
+    else
+      gemm_tn<double, GemmF64Wide>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skd);
   };
 
   // ---- initial state
@@ -248,19 +258,13 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       }
     });
     // exact marginals of the absorbed plan (FP64 MFMA)
-    gemm_tn<double>(
-        st, M, nact, Nc, KdT, ldn, Bd, ldn,
-        [=] __device__(int i, int t, double v) {
-          if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
-        },
-        act, nullptr, skd);
-    gemm_tn<double>(
-        st, Nc, nact, M, Kd, ldm, Ad, ldm,
-        [=] __device__(int j, int t, double v) {
-          const int p = pos[j + (long)t * Nc];
-          if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
-        },
-        act, nullptr, skd);
+    f64_product(M, Nc, KdT, ldn, Bd, ldn, [=] __device__(int i, int t, double v) {
+      if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
+    });
+    f64_product(Nc, M, Kd, ldm, Ad, ldm, [=] __device__(int j, int t, double v) {
+      const int p = pos[j + (long)t * Nc];
+      if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
+    });
     hipLaunchKernelGGL(k_dense_err, dim3(B), dim3(256), 0, st, M, B, a, r0, col_ptr, b, c0,
                        err_mode, tol, iter, maxiter, switch_thr, done, mode, iters, errs,
                        ndone);

@@ -88,8 +88,9 @@ template <> struct Mfma<float> {
 // DMA = 1: the tiles go from global memory straight into LDS
 // (global_load_lds_dwordx4, no register staging) through PF + 1 LDS stages, PF
 // k-steps ahead, in an unpadded layout whose 16-byte chunks are XOR-swizzled
-// by row (PAD is then unused). Needs 16-bit elements, K a multiple of BK and
-// 16-byte aligned rows; gemm_tn falls back to the register path otherwise.
+// by row (PAD is then unused). Needs tile rows of 128 or 256 bytes (BF16: 64
+// or 128 k; FP64: 16 or 32 k), K a multiple of BK and 16-byte aligned rows;
+// gemm_tn falls back to the register path otherwise.
 template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1,
           int DMA_ = 0>
 struct GemmShape {
@@ -102,11 +103,11 @@ struct GemmShape {
   static constexpr int LDL = BK + PAD;
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave block must be a multiple of 16");
   static_assert(BK % Mfma<T>::KI == 0, "k-step must be a multiple of the MFMA k");
-  static_assert(!DMA || (sizeof(T) == 2 && (BK == 64 || BK == 128) &&
-                         BM % (WGM * WGN * 8 * 64 / BK) == 0 &&
-                         BN % (WGM * WGN * 8 * 64 / BK) == 0),
-                "the direct-to-LDS path takes 16-bit tiles of 64 or 128 k (8 or 16 chunks of "
-                "16 bytes a row) whose rows split into 1 KB moves per wavefront");
+  static_assert(!DMA || ((BK * sizeof(T) == 128 || BK * sizeof(T) == 256) &&
+                         BM % (WGM * WGN * 1024 / (BK * sizeof(T))) == 0 &&
+                         BN % (WGM * WGN * 1024 / (BK * sizeof(T))) == 0),
+                "the direct-to-LDS path takes tile rows of 128 or 256 bytes (8 or 16 chunks "
+                "of 16 bytes) that split into 1 KB moves per wavefront");
 };
 
 template <typename T> struct GemmCfg;
@@ -115,6 +116,15 @@ template <> struct GemmCfg<float> : GemmShape<float, 128, 128, 16, 4, 2, 2> {};
 // moves in flight (4 LDS stages, 128 KB)
 template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 4, 3, 1> {};
 template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
+
+// FP64 products with many columns (>= 256): 128 x 128 tiles, 16 wavefronts,
+// direct-to-LDS moves three k-steps ahead (60.8 TFLOP/s on 4096 x 1024 x 4096
+// against 55 for the 64 x 64 register tiles). Each output element sees the
+// same MFMA sequence as with GemmCfg<double>, so results are identical.
+using GemmF64Wide = GemmShape<double, 128, 128, 16, 2, 4, 4, 3, 1>;
+// the tiles of a product over T with many columns
+template <typename T> struct WideCfg { using type = GemmCfg<T>; };
+template <> struct WideCfg<double> { using type = GemmF64Wide; };
 
 // Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix
 // stored with its k values contiguous (element (r, k) at p[r * ld + k]) into
@@ -376,6 +386,7 @@ __device__ __forceinline__ void gemm_main_dma(
   constexpr int NW = CF::WGM * CF::WGN;
   constexpr int ROWB = BK * (int)sizeof(T);  // bytes of a tile row: 8 or 16 chunks
   constexpr int CPR = ROWB / 16;             // chunks per row
+  constexpr int EPC = 16 / (int)sizeof(T);   // elements per chunk
   constexpr int RPM = 64 / CPR;              // rows per move (64 lanes x 16 bytes)
   constexpr int PF = CF::PF, STAGES = PF + 1;
   constexpr int STAGEB = (BM + BN) * ROWB;
@@ -392,13 +403,13 @@ __device__ __forceinline__ void gemm_main_dma(
 #pragma unroll
   for (int q = 0; q < MA; ++q) {
     const int rl = (wv * MA + q) * RPM + sub, r = m0 + rl;
-    pa[q] = A + (long)(r < M ? r : 0) * lda + (slot ^ (rl & 7)) * 8;
+    pa[q] = A + (long)(r < M ? r : 0) * lda + (slot ^ (rl & 7)) * EPC;
   }
 #pragma unroll
   for (int q = 0; q < MB; ++q) {
     const int rl = (wv * MB + q) * RPM + sub, n = n0 + rl;
     const long r = n < N ? (nmap ? nmap[n] : n) : (nmap ? nmap[0] : 0);
-    pb[q] = B + r * ldb + (slot ^ (rl & 7)) * 8;
+    pb[q] = B + r * ldb + (slot ^ (rl & 7)) * EPC;
   }
   const int nst = (kend - kbeg) / BK;
   auto issue = [&](int s) {
@@ -414,10 +425,12 @@ __device__ __forceinline__ void gemm_main_dma(
 #pragma unroll
   for (int q = 0; q < PF; ++q)
     if (q < nst) issue(q);
-  const int lr = lane & 15, lc = (lane >> 4) * MF::KL / 8;  // the lane's chunk within a k-block
-  auto frag = [&](const RWIG_LDS char *tile, int row, int c) -> typename MF::frag_t {
-    return *reinterpret_cast<const RWIG_LDS typename MF::frag_t *>(tile + row * ROWB +
-                                                                   ((c ^ (row & 7)) << 4));
+  const int lr = lane & 15, lk = (lane >> 4) * MF::KL;  // the lane's row and first k
+  // fragment of tile row `row` at k: chunk k / EPC (at its swizzled slot),
+  // element k % EPC within it
+  auto frag = [&](const RWIG_LDS char *tile, int row, int k) -> typename MF::frag_t {
+    return *reinterpret_cast<const RWIG_LDS typename MF::frag_t *>(
+        tile + row * ROWB + (((k / EPC) ^ (row & 7)) << 4) + (k % EPC) * (int)sizeof(T));
   };
   for (int s = 0; s < nst; ++s) {
     // this wavefront's moves of step s have landed once only the later
@@ -445,12 +458,12 @@ __device__ __forceinline__ void gemm_main_dma(
     const RWIG_LDS char *as = lds + (s % STAGES) * STAGEB, *bs = as + BM * ROWB;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += MF::KI) {
-      const int c = kk / 8 + lc;
+      const int k = kk + lk;
       typename MF::frag_t af[TM], bfr[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = frag(as, wm * WM + i * 16 + lr, c);
+      for (int i = 0; i < TM; ++i) af[i] = frag(as, wm * WM + i * 16 + lr, k);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bfr[j] = frag(bs, wn * WN + j * 16 + lr, c);
+      for (int j = 0; j < TN; ++j) bfr[j] = frag(bs, wn * WN + j * 16 + lr, k);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -563,7 +576,8 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
     // the direct-to-LDS path moves whole 16-byte chunks of whole k-steps: other
     // shapes take the register path with the same tiles (same arithmetic,
     // same split workspace)
-    const bool ok = K % CF::BK == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+    constexpr long EPC = 16 / sizeof(T);
+    const bool ok = K % CF::BK == 0 && lda % EPC == 0 && ldb % EPC == 0 &&
                     (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
                     (reinterpret_cast<uintptr_t>(B) & 15) == 0;
     if (!ok) {

@@ -19,6 +19,9 @@
 
 namespace rwig {
 
+// tiles of the WDL products (S * D columns: the wide FP64 tiles)
+template <typename T> using WdlCfg = typename WideCfg<T>::type;
+
 namespace {
 
 template <typename T> struct WdlBuf {
@@ -97,13 +100,13 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
     T *kvb = KVB(l), *ub1 = UB(l + 1), *ktub1 = KTUB(l + 1), *vb1 = VB(l + 1);
     T *bb1 = bB(l + 1);
     // KVB = K VB_l ; UB_{l+1} = A / KVB
-    gemm_tn<T>(st, N, SD, N, KT, ldk, VB(l), ldn, [=] __device__(int i, int c, T v) {
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, VB(l), ldn, [=] __device__(int i, int c, T v) {
       const long k = i + (long)c * ldn;
       kvb[k] = v;
       ub1[k] = A[i + (long)(c % S) * N] / v;
     });
     // KTUB = K^T UB_{l+1}
-    gemm_tn<T>(st, N, SD, N, K, ldk, ub1, ldn,
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, K, ldk, ub1, ldn,
                [=] __device__(int j, int c, T v) { ktub1[j + (long)c * ldn] = v; });
     // bB = prod_s KTUB^w ; VB = bB / KTUB
     launch_map(st, (long)N * D, [=] __device__(long k) {
@@ -152,7 +155,7 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
       });
     } else {
       // VBbar = -K^T Xb
-      gemm_tn<T>(st, N, SD, N, K, ldk, Xb, ldn,
+      gemm_tn<T, WdlCfg<T>>(st, N, SD, N, K, ldk, Xb, ldn,
                  [=] __device__(int j, int c, T v) { VBbar[j + (long)c * ldn] = -v; });
       // bBbar = rowsum(VBbar / KTUB); tmp = (bBbar w^T - VBbar / KTUB) .* VB_l
       launch_map(st, (long)N * D, [=] __device__(long k) {
@@ -170,7 +173,7 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
       });
     }
     // UBbar = K tmp ; ABbar += UBbar / KVB_{l-1} ; Xb = (UBbar .* UB_l) / KVB_{l-1}
-    gemm_tn<T>(st, N, SD, N, KT, ldk, tmp, ldn, [=] __device__(int i, int c, T v) {
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, tmp, ldn, [=] __device__(int i, int c, T v) {
       const long k = i + (long)c * ldn;
       ABbar[k] += v / kvblm1[k];
       Xb[k] = (v * ubl[k]) / kvblm1[k];
@@ -280,7 +283,7 @@ void wdl_infer(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, 
   launch_map(st, MSD, [=] __device__(long k) { V[k] = (k % ldn < N) ? T(1) : T(0); });
   launch_map(st, (long)N * Dn, [=] __device__(long k) { bb[k] = T(0); });
   launch_map(st, (long)Dn, [=] __device__(long d) { done[d] = 0; });
-  gemm_tn<T>(st, N, SD, N, KT, ldk, V, ldn,
+  gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, V, ldn,
              [=] __device__(int i, int c, T v) { KV[i + (long)c * ldn] = v; });
   for (int it = 0; it < maxiter; ++it) {
     // U = A / KV (running documents)
@@ -289,7 +292,7 @@ void wdl_infer(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, 
       if (done[c / S]) return;
       U[i + c * ldn] = A[i + (c % S) * N] / KV[i + c * ldn];
     });
-    gemm_tn<T>(st, N, SD, N, K, ldk, U, ldn,
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, K, ldk, U, ldn,
                [=] __device__(int j, int c, T v) { KTU[j + (long)c * ldn] = v; });
     launch_map(st, (long)N * Dn, [=] __device__(long k) {
       const long i = k % N, d = k / N;
@@ -299,7 +302,7 @@ void wdl_infer(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, 
       bb[k] = p;
       for (int s = 0; s < S; ++s) V[i + (d * S + s) * ldn] = p / KTU[i + (d * S + s) * ldn];
     });
-    gemm_tn<T>(st, N, SD, N, KT, ldk, V, ldn, [=] __device__(int i, int c, T v) {
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, V, ldn, [=] __device__(int i, int c, T v) {
       if (!done[c / S]) KV[i + (long)c * ldn] = v;
     });
     // per-document err = ||U .* KV - A||_F ; stop when err < tol

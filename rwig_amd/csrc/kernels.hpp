@@ -83,16 +83,33 @@ k_gemv_n_part(const double *__restrict__ K, int M, int N, long ldk,
 }
 
 // out index k = s*M + i: op(k, sum over chunks of part)
+// Chunk merges. The partial results of the row passes (nchunk chunks per
+// output element, element k of chunk c at c * len + k) are merged by
+// workgroups of 8 wavefronts: wavefront w takes chunks w, w + 8, ... of 64
+// consecutive elements (coalesced loads, 8 times the parallelism of one
+// thread per element), then wavefront 0 combines the 8 results through LDS.
+// The order is fixed: a result depends on the shape only.
+constexpr int kMergeWaves = 8, kMergeTB = kWave * kMergeWaves;
+
+inline unsigned merge_grid(long len) { return (unsigned)((len + kWave - 1) / kWave); }
+
 template <typename T, class Op>
-__global__ void __launch_bounds__(kTB)
+__global__ void __launch_bounds__(kMergeTB)
 k_reduce_part(const T *__restrict__ part, int nchunk, long len, Op op,
               const LoopState *ls) {
   if (ls && !ls->active) return;
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
-       k += (long)gridDim.x * blockDim.x) {
-    T s = T(0);
-    for (int c = 0; c < nchunk; ++c) s += part[(long)c * len + k];
-    op(k, s);
+  __shared__ T sh[kMergeWaves][kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const long k = blockIdx.x * (long)kWave + lane;
+  T v = T(0);
+  if (k < len)
+    for (int c = w; c < nchunk; c += kMergeWaves) v += part[(long)c * len + k];
+  sh[w][lane] = v;
+  __syncthreads();
+  if (w == 0 && k < len) {
+    T r = sh[0][lane];
+    for (int u = 1; u < kMergeWaves; ++u) r += sh[u][lane];
+    op(k, r);
   }
 }
 
@@ -113,8 +130,8 @@ void gemv_n(hipStream_t st, const double *K, int M, int N, long ldk,
   }
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL((k_reduce_part<double, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0,
-                     st, part, ch.nchunk, len, op, ls);
+  hipLaunchKernelGGL((k_reduce_part<double, Op>), dim3(merge_grid(len)), dim3(kMergeTB), 0, st,
+                     part, ch.nchunk, len, op, ls);
   check_launch();
 }
 
@@ -433,18 +450,27 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
 }
 
 template <typename T, class Op>
-__global__ void __launch_bounds__(kTB)
+__global__ void __launch_bounds__(kMergeTB)
 k_lse_row_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, long len, T reg,
                 Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
+  __shared__ T shm[kMergeWaves][kWave], sht[kMergeWaves][kWave];
   const T back = T((double)reg * kLn2);
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < len;
-       k += (long)gridDim.x * blockDim.x) {
-    SoftMin<T> a;
-    a.m = pm[k];
-    a.t = ps[k];
-    for (int c = 1; c < nchunk; ++c) a.merge(pm[(long)c * len + k], ps[(long)c * len + k]);
-    op(k, a.value(back));
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const long k = blockIdx.x * (long)kWave + lane;
+  SoftMin<T> a;
+  a.init();
+  if (k < len)
+    for (int c = w; c < nchunk; c += kMergeWaves) a.merge(pm[(long)c * len + k], ps[(long)c * len + k]);
+  shm[w][lane] = a.m;
+  sht[w][lane] = a.t;
+  __syncthreads();
+  if (w == 0 && k < len) {
+    SoftMin<T> b;
+    b.m = shm[0][lane];
+    b.t = sht[0][lane];
+    for (int u = 1; u < kMergeWaves; ++u) b.merge(shm[u][lane], sht[u][lane]);
+    op(k, b.value(back));
   }
 }
 
@@ -461,8 +487,8 @@ void lse_row(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, lon
                                      ldc, F, ldf, G, ldg, S, (T)reg, ch.cw, pm, ps, ls));
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL((k_lse_row_merge<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
-                     pm, ps, ch.nchunk, len, (T)reg, op, ls);
+  hipLaunchKernelGGL((k_lse_row_merge<T, Op>), dim3(merge_grid(len)), dim3(kMergeTB), 0, st, pm,
+                     ps, ch.nchunk, len, (T)reg, op, ls);
   check_launch();
 }
 
@@ -643,8 +669,8 @@ void wrow(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long l
                                      ls));
   check_launch();
   const long len = (long)S * M;
-  hipLaunchKernelGGL((k_reduce_part<T, Op>), dim3(grid_for(len, kTB, 4096)), dim3(kTB), 0, st,
-                     part, ch.nchunk, len, op, ls);
+  hipLaunchKernelGGL((k_reduce_part<T, Op>), dim3(merge_grid(len)), dim3(kMergeTB), 0, st, part,
+                     ch.nchunk, len, op, ls);
   check_launch();
 }
 
@@ -686,26 +712,37 @@ void transpose(hipStream_t st, const T *src, int rows, int cols, long lds, T *ds
 // R_s(i,j) = (C(i,j) - G(j,s)) - F(i,s) here (the row kernels' order of the
 // two subtractions), equal to the column kernels' up to rounding.
 
-// res[s] of column j from the chunk partials of a row pass over CT. A block
-// covers 256 / SM columns: thread (column, slice) merges that slice's chunks
-// into LDS, then one thread per column calls op with the column's S values.
+// res[s] of column j from the chunk partials of a row pass over CT. A
+// workgroup covers 64 / SM columns: lane (column, slice) of wavefront w merges
+// chunks w, w + 8, ..., wavefront 0 combines the 8 states into LDS, then one
+// thread per column calls op with the column's S values.
 template <typename T, int SM, class Op>
-__global__ void __launch_bounds__(kTB)
+__global__ void __launch_bounds__(kMergeTB)
 k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, int N, int S,
                 T reg, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  constexpr int JB = kTB / SM;
+  constexpr int JB = kWave / SM;  // columns per workgroup
+  __shared__ T shm[kMergeWaves][kWave], sht[kMergeWaves][kWave];
   __shared__ T r[JB][SM];
   const T back = T((double)reg * kLn2);
-  const int jj = threadIdx.x / SM, s = threadIdx.x % SM;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int jj = lane / SM, s = lane % SM;
   const int j = blockIdx.x * JB + jj;
-  if (j < N && s < S) {
-    SoftMin<T> a;
-    const long k = (long)s * N + j, len = (long)S * N;
-    a.m = pm[k];
-    a.t = ps[k];
-    for (int c = 1; c < nchunk; ++c) a.merge(pm[c * len + k], ps[c * len + k]);
-    r[jj][s] = a.value(back);
+  const bool valid = j < N && s < S;
+  const long k = (long)s * N + j, len = (long)S * N;
+  SoftMin<T> a;
+  a.init();
+  if (valid)
+    for (int c = w; c < nchunk; c += kMergeWaves) a.merge(pm[c * len + k], ps[c * len + k]);
+  shm[w][lane] = a.m;
+  sht[w][lane] = a.t;
+  __syncthreads();
+  if (w == 0 && valid) {
+    SoftMin<T> b;
+    b.m = shm[0][lane];
+    b.t = sht[0][lane];
+    for (int u = 1; u < kMergeWaves; ++u) b.merge(shm[u][lane], sht[u][lane]);
+    r[jj][s] = b.value(back);
   }
   __syncthreads();
   if (threadIdx.x < JB) {
@@ -715,18 +752,26 @@ k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, 
 }
 
 template <typename T, int SM, class Op>
-__global__ void __launch_bounds__(kTB)
+__global__ void __launch_bounds__(kMergeTB)
 k_sum_cols(const T *__restrict__ part, int nchunk, int N, int S, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  constexpr int JB = kTB / SM;
+  constexpr int JB = kWave / SM;
+  __shared__ T sh[kMergeWaves][kWave];
   __shared__ T r[JB][SM];
-  const int jj = threadIdx.x / SM, s = threadIdx.x % SM;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int jj = lane / SM, s = lane % SM;
   const int j = blockIdx.x * JB + jj;
-  if (j < N && s < S) {
-    const long k = (long)s * N + j, len = (long)S * N;
-    T v = part[k];
-    for (int c = 1; c < nchunk; ++c) v += part[c * len + k];
-    r[jj][s] = v;
+  const bool valid = j < N && s < S;
+  const long k = (long)s * N + j, len = (long)S * N;
+  T v = T(0);
+  if (valid)
+    for (int c = w; c < nchunk; c += kMergeWaves) v += part[c * len + k];
+  sh[w][lane] = v;
+  __syncthreads();
+  if (w == 0 && valid) {
+    T t = sh[0][lane];
+    for (int u = 1; u < kMergeWaves; ++u) t += sh[u][lane];
+    r[jj][s] = t;
   }
   __syncthreads();
   if (threadIdx.x < JB) {
@@ -750,8 +795,8 @@ void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F,
     hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg,
                        F, ldf, S, (T)reg, ch.cw, pm, ps, ls);
     check_launch();
-    hipLaunchKernelGGL((k_lse_col_merge<T, SM, Op>), dim3((N + kTB / SM - 1) / (kTB / SM)),
-                       dim3(kTB), 0, st, pm, ps, ch.nchunk, N, S, (T)reg, op, ls);
+    hipLaunchKernelGGL((k_lse_col_merge<T, SM, Op>), dim3((N + kWave / SM - 1) / (kWave / SM)),
+                       dim3(kMergeTB), 0, st, pm, ps, ch.nchunk, N, S, (T)reg, op, ls);
   });
   check_launch();
 }
@@ -767,8 +812,8 @@ void wcol_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F, lo
     hipLaunchKernelGGL((k_wrow_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg, F,
                        ldf, S, (T)reg, rshift, x, (const T *)nullptr, ch.cw, part, ls);
     check_launch();
-    hipLaunchKernelGGL((k_sum_cols<T, SM, Op>), dim3((N + kTB / SM - 1) / (kTB / SM)), dim3(kTB),
-                       0, st, part, ch.nchunk, N, S, op, ls);
+    hipLaunchKernelGGL((k_sum_cols<T, SM, Op>), dim3((N + kWave / SM - 1) / (kWave / SM)),
+                       dim3(kMergeTB), 0, st, part, ch.nchunk, N, S, op, ls);
   });
   check_launch();
 }

@@ -82,7 +82,6 @@ k_gemv_n_part(const double *__restrict__ K, int M, int N, long ldk,
     if (t < ns) part[((long)c * S + s0 + t) * M + i] = acc[t];
 }
 
-// out index k = s*M + i: op(k, sum over chunks of part)
 // Chunk merges. The partial results of the row passes (nchunk chunks per
 // output element, element k of chunk c at c * len + k) are merged by
 // workgroups of 8 wavefronts: wavefront w takes chunks w, w + 8, ... of 64
@@ -93,6 +92,7 @@ constexpr int kMergeWaves = 8, kMergeTB = kWave * kMergeWaves;
 
 inline unsigned merge_grid(long len) { return (unsigned)((len + kWave - 1) / kWave); }
 
+// op(k, sum over the chunks of part) for every output element k
 template <typename T, class Op>
 __global__ void __launch_bounds__(kMergeTB)
 k_reduce_part(const T *__restrict__ part, int nchunk, long len, Op op,

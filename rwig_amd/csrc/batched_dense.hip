@@ -108,6 +108,25 @@ k_dense_loss(const double *__restrict__ C, long ldc, int M, const int *__restric
   }
 }
 
+// Update on the supports from the split partials of a product whose rows are
+// the Nc columns of C and whose columns are the active targets (Z = K^T W):
+// one thread per support entry p reads its own element, op(p, t, j, z). The
+// product's own epilogue would visit all Nc x targets entries, 95% of them
+// off the supports.
+template <class CF, class Op>
+__global__ void __launch_bounds__(256)
+k_support(const float *__restrict__ work, int gx, long ntile, int nnz,
+          const int *__restrict__ tgt_of, const int *__restrict__ col_idx,
+          const int *__restrict__ colpos, const int *__restrict__ done, Op op) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= nnz) return;
+  const int t = tgt_of[p];
+  const int n = colpos[t];
+  if (n < 0 || done[t]) return;
+  const int j = col_idx[p];
+  op(p, t, j, work[tile_partial<CF>(j, n, 0, gx, ntile)]);
+}
+
 template <typename T> T *alloc_into(std::vector<DBuf<char>> &pool, size_t n) {
   pool.emplace_back(n * sizeof(T));
   return reinterpret_cast<T *>(pool.back().get());
@@ -163,6 +182,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
   int *ndone = alloc_into<int>(pool, 1);
   int *act = alloc_into<int>(pool, B);  // targets still iterating (GEMM column map)
+  int *colpos = alloc_into<int>(pool, B);  // target -> its column in act, or -1
   // GEMM operands
   uint16_t *W = alloc_into<uint16_t>(pool, (size_t)ldm * B);
   uint16_t *X = alloc_into<uint16_t>(pool, (size_t)ldn * B);
@@ -196,6 +216,24 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       gemm_tn<uint16_t, NarrowBf16>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
     else
       gemm_tn<uint16_t>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skb);
+  };
+  // Z = K^T W over the active targets, then op(p, t, j, z) on the support
+  // entries only, from the product's partial tiles (no k split)
+  auto ktw_support = [&](auto op) {
+    auto run = [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<uint16_t, CF>(st, Nc, nact, M, Kb, ldm, W, ldm,
+                            [] __device__(int, int, float) {}, act, nullptr, skb, false);
+      const int gx = (Nc + CF::BM - 1) / CF::BM;
+      const long ntile = (long)gx * ((nact + CF::BN - 1) / CF::BN);
+      hipLaunchKernelGGL((k_support<CF, decltype(op)>), dim3((nnz + 255) / 256), dim3(256), 0, st,
+                         skb.work, gx, ntile, nnz, tgt_of, col_idx, colpos, done, op);
+      check_launch();
+    };
+    if (nact <= kNarrowTargets)
+      run(NarrowBf16{});
+    else
+      run(GemmCfg<uint16_t>{});
   };
   // FP64 products: the wide tiles while many targets are active
   auto f64_product = [&](int rows, int kdim, const double *Am, long la, const double *Bm,
@@ -279,10 +317,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     });
     bf16_product(M, Nc, KbT, ldn, X, ldn,
                  [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; });
-    bf16_product(Nc, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
-      const int p = pos[j + (long)t * Nc];
-      if (p >= 0 && !done[t]) sb[p] = (float)(c0[p] / (double)z);
-    });
+    ktw_support([=] __device__(int p, int, int, float z) { sb[p] = (float)(c0[p] / (double)z); });
     // u = a / r0 with v = 1; W for the first product of the cycle
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
@@ -300,9 +335,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
 
   auto iterate = [&]() {
     // Z = K^T W -> v, X
-    bf16_product(Nc, M, Kb, ldm, W, ldm, [=] __device__(int j, int t, float z) {
-      const int p = pos[j + (long)t * Nc];
-      if (p < 0 || done[t]) return;
+    ktw_support([=] __device__(int p, int t, int j, float z) {
       double v;
       if (mode[t]) {
         v = b[p] / (c0[p] + (double)be[p] * (double)z);
@@ -332,8 +365,10 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
   // active list: every target at first; after each absorption the targets
   // that stopped leave the GEMMs
   std::vector<int> hact(B), hdone_v(B);
-  for (int t = 0; t < B; ++t) hact[t] = t;
+  std::vector<int> hcolpos(B);
+  for (int t = 0; t < B; ++t) hact[t] = hcolpos[t] = t;
   h2d(act, hact.data(), B, st);
+  h2d(colpos, hcolpos.data(), B, st);
   int iter = 0, hdone = 0;
   absorb(iter);
   while (true) {
@@ -344,9 +379,15 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       d2h(hdone_v.data(), done, B, st);
       RWIG_HIP_CHECK(hipStreamSynchronize(st));
       nact = 0;
-      for (int t = 0; t < B; ++t)
-        if (!hdone_v[t]) hact[nact++] = t;
+      for (int t = 0; t < B; ++t) {
+        hcolpos[t] = -1;
+        if (!hdone_v[t]) {
+          hcolpos[t] = nact;
+          hact[nact++] = t;
+        }
+      }
       h2d(act, hact.data(), nact, st);
+      h2d(colpos, hcolpos.data(), B, st);
     }
     const int n = std::min(absorb_every, maxiter - iter);
     for (int s = 0; s < n; ++s) iterate();

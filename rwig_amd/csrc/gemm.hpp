@@ -532,6 +532,15 @@ k_gemm_tn(int M, int N, int K, const T *__restrict__ A, long lda, const T *__res
 // stores are contiguous across a wavefront.
 constexpr int kEpiCols = 16;
 
+// Index in the split workspace of element (row m, column n) of split z, for a
+// product whose grid has gx row tiles and ntile tiles in all: partial tiles
+// are column-major within the tile.
+template <class CF>
+__host__ __device__ inline long tile_partial(int m, int n, int z, int gx, long ntile) {
+  const long tile = (long)(n / CF::BN) * gx + m / CF::BM;
+  return ((long)z * ntile + tile) * CF::BM * CF::BN + (long)(n % CF::BN) * CF::BM + m % CF::BM;
+}
+
 template <typename T, class CF, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
@@ -568,10 +577,13 @@ k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
 // of columns actually used, read by the kernel (N is then an upper bound that
 // sizes the grid), so that a launch can be recorded once in a graph and
 // replayed over a changing column set.
+// With a split workspace and epilogue = false, only the product runs: the
+// caller reads the partial tiles itself (tile_partial gives the index of an
+// element), for an update that visits fewer elements than M x N.
 template <typename T, class CF = GemmCfg<T>, class Op>
 void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T *B, long ldb,
              Op op, const int *nmap = nullptr, const int *ncount = nullptr,
-             const SplitK<T> &sk = SplitK<T>()) {
+             const SplitK<T> &sk = SplitK<T>(), bool epilogue = true) {
   if constexpr (CF::DMA) {
     // the direct-to-LDS path moves whole 16-byte chunks of whole k-steps: other
     // shapes take the register path with the same tiles (same arithmetic,
@@ -582,7 +594,7 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
                     (reinterpret_cast<uintptr_t>(B) & 15) == 0;
     if (!ok) {
       using Reg = GemmShape<T, CF::BM, CF::BN, CF::BK, 16, CF::WGM, CF::WGN, 2, 0>;
-      gemm_tn<T, Reg>(st, M, N, K, A, lda, B, ldb, op, nmap, ncount, sk);
+      gemm_tn<T, Reg>(st, M, N, K, A, lda, B, ldb, op, nmap, ncount, sk, epilogue);
       return;
     }
   }
@@ -596,7 +608,7 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
   hipLaunchKernelGGL((k_gemm_tn<T, CF, Op>), grid, dim3(CF::NT), 0, st, M, N, K, A, lda, B,
                      ldb, nmap, ncount, sk, op);
   check_launch();
-  if (sk.work) {
+  if (sk.work && epilogue) {
     hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, CF, Op>), dim3(grid.x, grid.y, BN / kEpiCols),
                        dim3(256), 0, st, M, N, nmap, ncount, sk.ks, sk.work, op);
     check_launch();

@@ -333,7 +333,9 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     });
   };
 
-  auto iterate = [&]() {
+  // keep: the last iteration before an absorption, whose u the absorption
+  // folds into f (the other iterations need only W, not u itself)
+  auto iterate = [&](bool keep) {
     // Z = K^T W -> v, X
     ktw_support([=] __device__(int p, int t, int j, float z) {
       double v;
@@ -358,7 +360,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
         u = a[i] / ((double)ra[k] * (double)y);
         W[i + (long)t * ldm] = f2bf((float)((double)al[k] * u));
       }
-      lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
+      if (keep) lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
     });
   };
 
@@ -390,7 +392,7 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
       h2d(colpos, hcolpos.data(), B, st);
     }
     const int n = std::min(absorb_every, maxiter - iter);
-    for (int s = 0; s < n; ++s) iterate();
+    for (int s = 0; s < n; ++s) iterate(s == n - 1);
     iter += n;
     absorb(iter);
   }

@@ -201,15 +201,17 @@ int rwig_gibbs_lowp_dev(const double *C, long ldc, int M, int N, double reg,
  * error <= zero_tol). Targets whose absorbed error is above switch_thr
  * iterate in a multiplicative form (positive sums only), the others in an
  * additive defect-correction form whose fixed point is the FP64 one.
- * The kernel handle holds exp(-C/reg) in BF16 and FP64 in both orientations
- * (rwig_dense_kernel_create, one per cost matrix and reg). tgt_of[p] is the
- * target owning support entry p. Other arguments as rwig_sinkhorn_batched_dev.
- * No counterpart in the reference (it solves one problem per call). */
+ * The kernel handle holds exp(-C/reg) in BF16 and FP64 in both orientations,
+ * and exp(-C/reg) .* C in FP64 for the transport cost (rwig_dense_kernel_create,
+ * one per cost matrix and reg); the solve reads the cost only through it.
+ * tgt_of[p] is the target owning support entry p. Other arguments as
+ * rwig_sinkhorn_batched_dev. No counterpart in the reference (it solves one
+ * problem per call). */
 int rwig_dense_kernel_create(const double *C, long ldc, int M, int Nc,
                              double reg, void *stream, void **handle);
 int rwig_dense_kernel_destroy(void *handle);
-int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc,
-                                   const double *a, int B, const int *col_ptr,
+int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *a, int B,
+                                   const int *col_ptr,
                                    const int *col_idx, const int *tgt_of,
                                    int nnz, const double *b, double reg,
                                    int max_iter, double zero_tol, int err_mode,

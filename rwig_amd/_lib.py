@@ -46,9 +46,8 @@ SIGNATURES = {
     "rwig_dense_kernel_create": ([_vp, ctypes.c_long, _i, _i, _d, _vp,
                                   ctypes.POINTER(ctypes.c_void_p)], _i),
     "rwig_dense_kernel_destroy": ([_vp], _i),
-    "rwig_sinkhorn_batched_mfma_dev": ([_vp, _vp, ctypes.c_long, _vp, _i, _vp, _vp, _vp, _i, _vp,
-                                        _d, _i, _d, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp],
-                                       _i),
+    "rwig_sinkhorn_batched_mfma_dev": ([_vp, _vp, _i, _vp, _vp, _vp, _i, _vp, _d, _i, _d, _i,
+                                        _i, _d, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "rwig_wdl": ([_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d, _i, _d, _i, _i, _i, _d, _d, _d, _d,
                   _d, _i, _i], _i),
     "rwig_wdl_init": ([_dp, _dp, _dp, _dp, _dp, _i, _i, _i, _d, _i, _d, _i, _i, _i, _d, _d, _d,

@@ -103,8 +103,9 @@ def sinkhorn_batched(C, a, targets: TargetBatch, reg, precision="fp64", max_iter
 
 
 class DenseKernel:
-    """exp(-C/reg) in BF16 and FP64, both orientations, for the MFMA path
-    (rwig_dense_kernel_create). Built once per cost matrix and reg."""
+    """exp(-C/reg) in BF16 and FP64, both orientations, and exp(-C/reg) .* C in
+    FP64, for the MFMA path (rwig_dense_kernel_create). Built once per cost
+    matrix and reg."""
 
     def __init__(self, C, reg, stream=None):
         if C.dtype != torch.float64 or C.stride(0) != 1:
@@ -147,7 +148,7 @@ def sinkhorn_batched_mfma(C, a, targets: TargetBatch, reg, kernel: DenseKernel =
             if with_loss else None,
         }
     check(lib().rwig_sinkhorn_batched_mfma_dev(
-        kernel.handle, _ptr(C), C.stride(1), _ptr(a), targets.B, _ptr(targets.col_ptr),
+        kernel.handle, _ptr(a), targets.B, _ptr(targets.col_ptr),
         _ptr(targets.col_idx), _ptr(targets.tgt_of), targets.nnz, _ptr(targets.b), float(reg),
         int(max_iter), float(zero_tol), ERR_MODES[err], int(absorb_every), float(switch_thr),
         _ptr(out["f"]), _ptr(out["g"]), _ptr(out["iter"]), _ptr(out["err"]), _ptr(out["loss"]),

@@ -81,30 +81,32 @@ k_dense_err(int M, int B, const double *__restrict__ a, const double *__restrict
   }
 }
 
-// transport cost and regularised loss of the absorbed plan, one block per target
+// transport cost and regularised loss of the absorbed plan P (marginals r0,
+// c0), one block per target: <P, C> = sum_i pc_i with pc = alpha .* ((K o C) beta),
+// and, since log P = (f + g - C) / reg,
+//   <P, C> + reg sum P (log P - 1) = f . r0 + g . c0 - reg sum r0
 __global__ void __launch_bounds__(256)
-k_dense_loss(const double *__restrict__ C, long ldc, int M, const int *__restrict__ col_ptr,
-             const int *__restrict__ col_idx, const double *__restrict__ F,
-             const double *__restrict__ G, double reg, double *loss) {
+k_dense_loss(int M, const int *__restrict__ col_ptr, const double *__restrict__ F,
+             const double *__restrict__ r0, const double *__restrict__ G,
+             const double *__restrict__ c0, const double *__restrict__ pc, double reg,
+             double *loss) {
   __shared__ double sh[256 / kWave];
   const int t = blockIdx.x;
-  double cc = 0.0, ce = 0.0;
-  for (int p = col_ptr[t]; p < col_ptr[t + 1]; ++p) {
-    const long col = col_idx[p];
-    const double gq = G[p];
-    for (int i = threadIdx.x; i < M; i += blockDim.x) {
-      const double cij = C[i + col * ldc];
-      const double rr = (cij - F[i + (long)t * M]) - gq;
-      const double pv = exp(-rr / reg);
-      cc += cij * pv;
-      ce += pv * (-rr / reg - 1.0);
-    }
+  double cc = 0.0, fr = 0.0, rs = 0.0, gc = 0.0;
+  for (int i = threadIdx.x; i < M; i += blockDim.x) {
+    const long k = i + (long)t * M;
+    cc += pc[k];
+    fr += F[k] * r0[k];
+    rs += r0[k];
   }
+  for (int p = col_ptr[t] + threadIdx.x; p < col_ptr[t + 1]; p += blockDim.x) gc += G[p] * c0[p];
   cc = block_sum(cc, sh);
-  ce = block_sum(ce, sh);
+  fr = block_sum(fr, sh);
+  rs = block_sum(rs, sh);
+  gc = block_sum(gc, sh);
   if (threadIdx.x == 0) {
     loss[2 * t] = cc;
-    loss[2 * t + 1] = cc + reg * ce;
+    loss[2 * t + 1] = (fr + gc) - reg * rs;
   }
 }
 
@@ -161,9 +163,26 @@ void dense_kernel_build(hipStream_t st, const double *C, long ldc, int M, int Nc
   });
 }
 
-void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const DenseKernel &dk,
-                            const double *a, int B, const int *col_ptr, const int *col_idx,
-                            const int *tgt_of, int nnz, const double *b, double reg,
+void dense_kernel_cost_build(hipStream_t st, const double *C, long ldc, double reg,
+                             DenseKernel &dk) {
+  const int M = dk.M, Nc = dk.Nc;
+  const long ldn = dk.ldn;
+  dk.KCT.reset((size_t)ldn * M);
+  double *KCT = dk.KCT.get();
+  // the same K(i,j) as KdT holds, times C(i,j)
+  launch_map(st, ldn * M, [=] __device__(long k) {
+    const long j = k % ldn, i = k / ldn;
+    double v = 0.0;
+    if (j < Nc) {
+      const double c = C[i + j * ldc];
+      v = exp(-c / reg) * c;
+    }
+    KCT[k] = v;
+  });
+}
+
+void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double *a, int B,
+                            const int *col_ptr, const int *col_idx, const int *tgt_of, int nnz, const double *b, double reg,
                             int maxiter, double tol, int err_mode, int absorb_every,
                             double switch_thr, double *F, double *G, int *iters, double *errs,
                             double *loss) {
@@ -397,8 +416,16 @@ void sinkhorn_batched_dense(hipStream_t st, const double *C, long ldc, const Den
     absorb(iter);
   }
   if (loss) {
-    hipLaunchKernelGGL(k_dense_loss, dim3(B), dim3(256), 0, st, C, ldc, M, col_ptr, col_idx, F,
-                       G, reg, loss);
+    // every target's final Ad, Bd, r0, c0 are those of its last absorption
+    for (int t = 0; t < B; ++t) hact[t] = t;
+    h2d(act, hact.data(), B, st);
+    nact = B;
+    double *pc = alloc_into<double>(pool, MB);
+    f64_product(M, Nc, dk.KCT.get(), ldn, Bd, ldn, [=] __device__(int i, int t, double v) {
+      pc[i + (long)t * M] = v * Ad[i + (long)t * ldm];
+    });
+    hipLaunchKernelGGL(k_dense_loss, dim3(B), dim3(256), 0, st, M, col_ptr, F, r0, G, c0, pc,
+                       reg, loss);
     check_launch();
   }
   RWIG_HIP_CHECK(hipStreamSynchronize(st));
@@ -417,6 +444,7 @@ int rwig_dense_kernel_create(const double *C, long ldc, int M, int Nc, double re
     auto *dk = new DenseKernel();
     try {
       dense_kernel_build((hipStream_t)stream, C, ldc, M, Nc, reg, *dk);
+      dense_kernel_cost_build((hipStream_t)stream, C, ldc, reg, *dk);
       RWIG_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
     } catch (...) {
       delete dk;
@@ -433,9 +461,9 @@ int rwig_dense_kernel_destroy(void *handle) {
   });
 }
 
-int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc, const double *a,
-                                   int B, const int *col_ptr, const int *col_idx,
-                                   const int *tgt_of, int nnz, const double *b, double reg,
+int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *a, int B, const int *col_ptr,
+                                   const int *col_idx, const int *tgt_of, int nnz,
+                                   const double *b, double reg,
                                    int max_iter, double zero_tol, int err_mode,
                                    int absorb_every, double switch_thr, double *f, double *g,
                                    int *iters, double *errs, double *loss, void *stream) {
@@ -444,10 +472,10 @@ int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *C, long ldc, cons
     RWIG_REQUIRE(B >= 1 && nnz >= B, "every target needs a support");
     RWIG_REQUIRE(reg > 0.0 && max_iter >= 0 && absorb_every >= 1, "invalid argument");
     RWIG_REQUIRE(err_mode == 0 || err_mode == 1, "err_mode must be 0 or 1");
-    RWIG_REQUIRE(C && a && col_ptr && col_idx && tgt_of && b && f && g && iters && errs,
+    RWIG_REQUIRE(a && col_ptr && col_idx && tgt_of && b && f && g && iters && errs,
                  "null argument");
     const DenseKernel &dk = *static_cast<const DenseKernel *>(handle);
-    sinkhorn_batched_dense((hipStream_t)stream, C, ldc, dk, a, B, col_ptr, col_idx, tgt_of, nnz,
+    sinkhorn_batched_dense((hipStream_t)stream, dk, a, B, col_ptr, col_idx, tgt_of, nnz,
                            b, reg, max_iter, zero_tol, err_mode, absorb_every, switch_thr, f, g,
                            iters, errs, loss);
   });

@@ -19,6 +19,10 @@ struct DenseKernel {
   long ldm = 0, ldn = 0;  // padded M, padded Nc (multiples of 8)
   DBuf<uint16_t> Kb, KbT;  // Kb: Kb[i + j*ldm] = K(i,j); KbT: KbT[j + i*ldn] = K(i,j)
   DBuf<double> Kd, KdT;
+  // KCT[j + i*ldn] = K(i,j) C(i,j): the transport cost of the batched solver as
+  // one FP64 product (built by dense_kernel_cost_build; empty in the serving
+  // engine, which needs only the regularised loss)
+  DBuf<double> KCT;
 };
 
 inline long pad8(long n) { return (n + 7) / 8 * 8; }
@@ -31,5 +35,9 @@ inline size_t dense_kernel_bytes(int M, int Nc) {
 // Fills dk from the device cost C (M x Nc, leading dimension ldc).
 void dense_kernel_build(hipStream_t st, const double *C, long ldc, int M, int Nc, double reg,
                         DenseKernel &dk);
+
+// Fills dk.KCT from the cost dk was built from.
+void dense_kernel_cost_build(hipStream_t st, const double *C, long ldc, double reg,
+                             DenseKernel &dk);
 
 }  // namespace rwig

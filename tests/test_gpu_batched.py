@@ -6,8 +6,9 @@ reduced problem (src/sinkhorn_impl_log.cpp), same iteration count, f and g to
 1e-10 relative. FP32 / BF16 modes: the returned potentials define a plan whose
 L1 marginal error, recomputed here in FP64 on the host, is below the stated
 tolerance, and whose transport cost agrees with the FP64 mode's to 1e-5
-relative. Every mode gives bit-identical results for a target solved alone and
-inside a batch.
+relative; the returned transport cost and regularised loss are those of that
+plan, evaluated here on the host. Every mode gives bit-identical results for a
+target solved alone and inside a batch.
 """
 import numpy as np
 import pytest
@@ -55,6 +56,17 @@ def plan_l1(C, a, sup, w, f, g, reg):
     return np.abs(P.sum(1) - a).sum() + np.abs(P.sum(0) - w).sum(), (Ct * P).sum()
 
 
+def plan_loss(C, sup, f, g, reg):
+    """Transport cost and regularised loss <P, C> + reg sum P (log P - 1) of the
+    plan the potentials define, with its terms' magnitude (for the tolerance)."""
+    Ct = C[:, sup]
+    lp = -((Ct - f[:, None]) - g[None, :]) / reg
+    P = np.exp(lp)
+    ent = reg * (P * (lp - 1.0))
+    cost = (Ct * P).sum()
+    return cost, cost + ent.sum(), (Ct * P).sum() + np.abs(ent).sum()
+
+
 @pytest.mark.parametrize("seed,M,B,lo,hi,reg", [
     (1, 300, 5, 8, 60, 0.05),
     (2, 1000, 3, 40, 90, 0.02),
@@ -98,6 +110,8 @@ def test_fast_modes_reach_tolerance_and_match_fp64(precision):
         assert abs(out[3][t] - e) <= 1e-9 + 1e-6 * e  # reported error is the plan's
         assert abs(c - c_ref) <= 1e-5 * abs(c_ref)
         assert abs(out[4][t, 0] - c) <= 1e-9 * abs(c)
+        _, rl, mag = plan_loss(C, sup[t], out[0][t], out[1][t], reg)
+        assert abs(out[4][t, 1] - rl) <= 1e-12 * mag, (t, out[4][t, 1], rl)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
@@ -148,6 +162,8 @@ def test_mfma_path_reaches_tolerance_and_matches_fp64(absorb_every):
         assert abs(out[3][t] - e) <= 1e-9 + 1e-6 * e
         assert abs(c - c_ref) <= 1e-5 * abs(c_ref)
         assert abs(out[4][t, 0] - c) <= 1e-9 * abs(c)
+        _, rl, mag = plan_loss(C, sup[t], out[0][t], out[1][t], reg)
+        assert abs(out[4][t, 1] - rl) <= 1e-12 * mag, (t, out[4][t, 1], rl)
 
 
 def test_mfma_path_alone_equals_in_mix():

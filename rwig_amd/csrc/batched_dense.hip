@@ -39,13 +39,14 @@ using NarrowBf16 = GemmShape<uint16_t, 64, 64, 64, 16, 2, 2, 2, 1>;
 constexpr int kNarrowTargets = 256;
 
 // per-target L1 (or reference) marginal error of the absorbed plan, stopping
-// decision, form of the next cycle; one block per target
+// decision, form of the next cycle; one block per target. cnt[0] counts the
+// targets stopped so far, cnt[1] the running ones in the multiplicative form.
 __global__ void __launch_bounds__(256)
 k_dense_err(int M, int B, const double *__restrict__ a, const double *__restrict__ r0,
             const int *__restrict__ col_ptr, const double *__restrict__ b,
             const double *__restrict__ c0, int err_mode, double tol, int iter, int maxiter,
             double switch_thr, int *done, int *mode, int *iters, double *errs,
-            int *ndone) {
+            int *cnt) {
   __shared__ double sh[256 / kWave];
   const int t = blockIdx.x;
   if (done[t]) return;
@@ -75,9 +76,10 @@ k_dense_err(int M, int B, const double *__restrict__ a, const double *__restrict
     iters[t] = iter;
     if (!(iter < maxiter && err >= tol)) {
       done[t] = 1;
-      atomicAdd(ndone, 1);
+      atomicAdd(&cnt[0], 1);
     }
     mode[t] = (err < switch_thr) ? 1 : 0;
+    if (!done[t] && !mode[t]) atomicAdd(&cnt[1], 1);
   }
 }
 
@@ -199,7 +201,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   double *c0 = alloc_into<double>(pool, nnz);
   int *pos = alloc_into<int>(pool, (size_t)Nc * B);
   int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
-  int *ndone = alloc_into<int>(pool, 1);
+  int *cnt = alloc_into<int>(pool, 2);  // targets stopped, multiplicative ones running
   int *act = alloc_into<int>(pool, B);  // targets still iterating (GEMM column map)
   int *colpos = alloc_into<int>(pool, B);  // target -> its column in act, or -1
   // GEMM operands
@@ -288,7 +290,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
     done[t] = 0;
     mode[t] = 0;
   });
-  launch_map(st, 1L, [=] __device__(long) { *ndone = 0; });
+  launch_map(st, 1L, [=] __device__(long) { cnt[0] = 0; });
 
   auto absorb = [&](int iter) {
     // fold the scalings into the potentials (targets still running)
@@ -322,26 +324,33 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
       const int p = pos[j + (long)t * Nc];
       if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
     });
+    launch_map(st, 1L, [=] __device__(long) { cnt[1] = 0; });
     hipLaunchKernelGGL(k_dense_err, dim3(B), dim3(256), 0, st, M, B, a, r0, col_ptr, b, c0,
-                       err_mode, tol, iter, maxiter, switch_thr, done, mode, iters, errs,
-                       ndone);
+                       err_mode, tol, iter, maxiter, switch_thr, done, mode, iters, errs, cnt);
     check_launch();
-    // calibration of the multiplicative form: the same BF16 products at u = v = 1
-    launch_map(st, MB, [=] __device__(long k) {
-      const long t = k / M, i = k % M;
-      W[i + t * ldm] = f2bf(al[k]);
-    });
-    launch_map(st, (long)nnz, [=] __device__(long p) {
-      X[col_idx[p] + (long)tgt_of[p] * ldn] = f2bf(be[p]);
-    });
-    bf16_product(M, Nc, KbT, ldn, X, ldn,
-                 [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; });
-    ktw_support([=] __device__(int p, int, int, float z) { sb[p] = (float)(c0[p] / (double)z); });
-    // u = a / r0 with v = 1; W for the first product of the cycle
+  };
+  // start of a cycle: u = a / r0 at v = 1, W for its first product. mult: some
+  // running target iterates in the multiplicative form, whose calibration
+  // ratios come from the same BF16 products at u = v = 1 (the additive form
+  // needs none)
+  auto start_cycle = [&](bool mult) {
+    if (mult) {
+      launch_map(st, MB, [=] __device__(long k) {
+        const long t = k / M, i = k % M;
+        W[i + t * ldm] = f2bf(al[k]);
+      });
+      launch_map(st, (long)nnz, [=] __device__(long p) {
+        X[col_idx[p] + (long)tgt_of[p] * ldn] = f2bf(be[p]);
+      });
+      bf16_product(M, Nc, KbT, ldn, X, ldn,
+                   [=] __device__(int i, int t, float y) { Yc[i + (long)t * M] = y; });
+      ktw_support(
+          [=] __device__(int p, int, int, float z) { sb[p] = (float)(c0[p] / (double)z); });
+    }
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
       if (done[t]) return;
-      ra[k] = (float)(r0[k] / (double)Yc[k]);
+      if (!mode[t]) ra[k] = (float)(r0[k] / (double)Yc[k]);
       const double u = a[i] / r0[k];
       lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
       W[i + t * ldm] = f2bf(mode[t] ? (float)((double)al[k] * (u - 1.0)) : (float)((double)al[k] * u));
@@ -390,11 +399,12 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   for (int t = 0; t < B; ++t) hact[t] = hcolpos[t] = t;
   h2d(act, hact.data(), B, st);
   h2d(colpos, hcolpos.data(), B, st);
-  int iter = 0, hdone = 0;
+  int iter = 0, hcnt[2] = {0, 0};
   absorb(iter);
   while (true) {
-    d2h(&hdone, ndone, 1, st);
+    d2h(hcnt, cnt, 2, st);
     RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    const int hdone = hcnt[0];
     if (hdone >= B || iter >= maxiter) break;
     if (hdone > B - nact) {
       d2h(hdone_v.data(), done, B, st);
@@ -410,6 +420,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
       h2d(act, hact.data(), nact, st);
       h2d(colpos, hcolpos.data(), B, st);
     }
+    start_cycle(hcnt[1] > 0);
     const int n = std::min(absorb_every, maxiter - iter);
     for (int s = 0; s < n; ++s) iterate(s == n - 1);
     iter += n;

@@ -178,3 +178,27 @@ def test_mfma_path_alone_equals_in_mix():
         assert np.array_equal(one[0][0], mix[0][t])
         assert np.array_equal(one[1][0], mix[1][t])
         assert one[2][0] == mix[2][t]
+
+
+@pytest.mark.parametrize("M", [256, 300])
+def test_mfma_path_same_result_in_every_tile_shape(M):
+    """900 targets pass through every tile shape of the products as they stop
+    (128 x 128 above 768 or 512 targets, then the smaller shapes); a target
+    solved alone runs in the smallest ones from its first iteration. Its
+    result must be the same bit for bit. M = 256 takes the direct-to-LDS
+    tiles, M = 300 (k not a multiple of the k-step) the register-path ones."""
+    C, a, sup, wts, reg = make(31, M, 900, 8, 40, 0.05)
+    Cd, ad = to_dev(C, a)
+    kern = RB.DenseKernel(Cd, reg)
+    tb = RB.TargetBatch(sup, wts)
+    mix = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, kernel=kern, absorb_every=10), tb)
+    order = np.argsort(mix[2])
+    print("iterations min/median/max", mix[2].min(), np.median(mix[2]), mix[2].max(), flush=True)
+    assert mix[2].max() > mix[2].min()  # targets stop at different absorptions
+    for t in [int(order[0]), int(order[len(order) // 2]), int(order[-1])]:
+        tb1 = RB.TargetBatch([sup[t]], [wts[t]])
+        one = host(RB.sinkhorn_batched_mfma(Cd, ad, tb1, reg, kernel=kern, absorb_every=10), tb1)
+        assert one[2][0] == mix[2][t]
+        assert np.array_equal(one[0][0], mix[0][t]), t
+        assert np.array_equal(one[1][0], mix[1][t]), t
+        assert np.array_equal(one[4][0], mix[4][t]), t

@@ -236,6 +236,15 @@ template <typename T> struct SoftMin {
     m = mn;
     t = a + b;
   }
+  // a group of values whose minimum mg is known: one rescale of the state to
+  // the new minimum, then one exponential per value (no compare or select)
+  template <int G> __device__ void add_group(T mg, const T (&r)[G]) {
+    const T mn = fmin(m, mg);
+    if (m != mn) t *= sm_exp2(mn - m);
+    m = mn;
+#pragma unroll
+    for (int u = 0; u < G; ++u) t += sm_exp2(m - r[u]);
+  }
   // the soft-min in the caller's units: (m - log2 t) / sc
   __device__ T value(T back) const { return (m - log2(t)) * back; }
 };
@@ -379,6 +388,21 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, i
 // Row soft-min partials over the column chunk blockIdx.y: pm = m, ps = t of
 // the chunk, at [(c * S + s) * M + i]. Slices s >= S (up to SM) run on zeros
 // and are not written, which keeps the inner loop free of per-slice branches.
+// The row's own shift f is constant along the row: the state runs on
+// x * sc - g and f is taken off the partial's minimum at the end. In FP32 the
+// columns go in groups of RowGroup: first the group's minimum, then one
+// exponential per value against it (SoftMin::add_group), about 4 VALU
+// operations and one exponential per value and slice instead of 7 and one for
+// the online update (row pass at the C3 shape 47 -> 37 us). FP64 keeps the
+// online update: its exponential is a long instruction sequence, which the
+// group's extra rescale and unrolled code make slower, not faster.
+template <typename T, int SM> struct RowGroup {
+  // columns per group: their values and one slice's terms stay in registers;
+  // the rescale costs one exponential per group and slice
+  static constexpr int G = sizeof(T) == 4 ? (SM <= 8 ? 16 : 8) : kColUnroll;
+  static constexpr bool two_step = sizeof(T) == 4;
+};
+
 template <typename T, int SM>
 __global__ void __launch_bounds__(kTB)
 k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
@@ -392,13 +416,10 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
   const int j0 = c * cw, j1 = min(N, j0 + cw);
   const T sc = T(kLog2e / (double)reg);
   const bool row = i < M;
-  T f[SM];
   SoftMin<T> st[SM];
 #pragma unroll
-  for (int s = 0; s < SM; ++s) {
-    st[s].init();
-    f[s] = (row && s < S) ? F[i + (long)s * ldf] * sc : T(0);
-  }
+  for (int s = 0; s < SM; ++s) st[s].init();
+  constexpr int GR = RowGroup<T, SM>::G;
   __shared__ T gs[kColTile * SM];
   for (int t0 = j0; t0 < j1; t0 += kColTile) {
     const int nc = min(kColTile, j1 - t0);
@@ -406,26 +427,40 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
     stage_cols<T, SM>(G, ldg, t0, nc, S, gs, sc);
     __syncthreads();
     if (row) {
-      // kColUnroll columns' loads issued ahead of their updates
       int jj = w;
-      for (; jj + (kColUnroll - 1) * kWavesPerBlock < nc; jj += kColUnroll * kWavesPerBlock) {
-        T x[kColUnroll];
+      // GR columns' loads in flight together, then their updates
+      for (; jj + (GR - 1) * kWavesPerBlock < nc; jj += GR * kWavesPerBlock) {
+        T x[GR];
 #pragma unroll
-        for (int u = 0; u < kColUnroll; ++u)
-          x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+        for (int u = 0; u < GR; ++u) x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+        if constexpr (RowGroup<T, SM>::two_step) {
 #pragma unroll
-        for (int u = 0; u < kColUnroll; ++u) {
-          const T *g = gs + (jj + u * kWavesPerBlock) * SM;
-          const T xs = x[u] * sc;
+          for (int s = 0; s < SM; ++s) {
+            T r[GR];
+            T mg = T(INFINITY);
 #pragma unroll
-          for (int s = 0; s < SM; ++s) st[s].add((xs - f[s]) - g[s]);
+            for (int u = 0; u < GR; ++u) {
+              r[u] = x[u] * sc - gs[(jj + u * kWavesPerBlock) * SM + s];
+              mg = fmin(mg, r[u]);
+            }
+            st[s].add_group(mg, r);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < GR; ++u) {
+            const T *g = gs + (jj + u * kWavesPerBlock) * SM;
+            const T xs = x[u] * sc;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) st[s].add(xs - g[s]);
+          }
         }
       }
+      // the last columns of the tile, one at a time
       for (; jj < nc; jj += kWavesPerBlock) {
         const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
         const T *g = gs + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s) st[s].add((xs - f[s]) - g[s]);
+        for (int s = 0; s < SM; ++s) st[s].add(xs - g[s]);
       }
     }
   }
@@ -443,7 +478,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
       a.t = sht[0][lane];
       for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane]);
       const long o = ((long)c * S + s) * M + i;
-      pm[o] = a.m;
+      pm[o] = a.m - F[i + (long)s * ldf] * sc;
       ps[o] = a.t;
     }
   }

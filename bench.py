@@ -56,9 +56,10 @@ def log(msg):
 
 
 def c2_gemm(torch, dev, stream):
-    """The dominant kernel of the MFMA path at its full shape: K^T W with
-    M = K = 4096 (the cost's points) and N = 1024 (the targets), BF16 inputs,
-    deferred epilogue. Returns (launch, flops per launch)."""
+    """The dominant kernel of the MFMA path at its full shape: the BF16 product
+    with M = K = 4096 (the cost's points) and N = 1024 (the targets), FP32
+    accumulation, deferred epilogue (as the path runs Y = K X). Returns
+    (launch, flops per launch)."""
     from rwig_amd._lib import check, lib
     gM, gN, gK = 4096, 1024, 4096
     Wt = torch.randn((gN, gK), device=dev).to(torch.bfloat16)
@@ -212,6 +213,51 @@ def c4_leg(torch, dev, stream, log):
     return out
 
 
+def bary_leg(log, eps=0.01, sample=4):
+    """Batched barycenters at L1 marginal error 1e-6 against the plain FP64
+    form: one cost (2048 points uniform in [0,1]^2, squared Euclidean), 256
+    problems of S = 4 histograms, eps 0.01. The batched form is the serving
+    engine with all 256 submitted at once (BF16 MFMA, FP64 absorption every 20,
+    stop at L1 marginal error 1e-6, checked per result). The plain form is the
+    reference's log barycenter in FP64 (rwig_barycenter_log, host buffers) on
+    a sample of the same problems, run for the iterations the batched form
+    needed on each, so that both do the same iterations per solve."""
+    import rwig_amd
+    from rwig_amd import workloads as W
+    from rwig_amd.serve import ServeSession
+    C, probs = W.bary_batch()
+    n, S = C.shape[0], probs[0][0].shape[1]
+    out = {"workload": f"batched barycenters: n={n} points (squared Euclidean), {len(probs)} "
+                       f"problems of S={S} Dirichlet(1) histograms, eps={eps}, L1 marginal "
+                       "err <= 1e-6"}
+    with ServeSession(hbm_budget_bytes=64 << 30, max_group_columns=1024, absorb_every=20) as s:
+        cid = s.register_cost(C)
+        s.result(s.submit_barycenter(cid, probs[0][0], probs[0][1], eps))  # warm-up
+        t = time.perf_counter()
+        rids = [s.submit_barycenter(cid, A, w, eps) for A, w in probs]
+        res = [s.result(r) for r in rids]
+        sps = len(probs) / (time.perf_counter() - t)
+    its = np.array([r["iter"] for r in res])
+    ok = all(r["err"] <= 1e-6 for r in res)
+    log(f"batched barycenters: {sps:.1f} solves/s, iterations min/mean/max {its.min()}/"
+        f"{its.mean():.1f}/{its.max()}, all converged {ok}")
+    t_plain = 0.0
+    for q in range(sample):
+        A, w = probs[q]
+        ctrl = {"method": "log", "reg": eps, "max_iter": int(its[q]), "zero_tol": 0.0}
+        t1 = time.perf_counter()
+        rwig_amd.barycenter(A, C, w, barycenter_control=ctrl)
+        t_plain += time.perf_counter() - t1
+    plain_sps = sample / t_plain
+    log(f"plain FP64 log barycenter per call: {plain_sps:.2f} solves/s over {sample} problems")
+    out.update(bf16_mfma_solves_per_s=sps, all_converged=ok,
+               iterations_mean=float(its.mean()), fp64_plain_per_call_solves_per_s=plain_sps,
+               plain_sample=sample, speedup_vs_plain_fp64=sps / plain_sps,
+               plain_note="rwig_barycenter_log per call (host buffers, C uploaded each call), "
+                          "max_iter = the batched form's iterations for the same problem")
+    return out
+
+
 def c5_leg(log, paced_s=8.0, graphs=True):
     """Serving mix: 32 cost matrices (n log-uniform in [64, 8192]) resident,
     70% Sinkhorn / 30% barycenter (S in {2, 4, 8}), eps in {0.1, 0.01}, stop at
@@ -288,7 +334,7 @@ def main():
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant product (5 launches) and a 256 MiB copy (5 "
                          "launches): the workload of the HBM-traffic counter runs")
-    ap.add_argument("--only", choices=("c3", "lse", "c4", "c5"),
+    ap.add_argument("--only", choices=("c3", "lse", "c4", "c5", "bary"),
                     help="run one secondary measurement alone and print its JSON object "
                          "(the workload of a profiler run)")
     ap.add_argument("--serve-eager", action="store_true",
@@ -308,7 +354,8 @@ def main():
         stream = torch.cuda.current_stream()
         legs = {"c3": lambda: c3_leg(log), "lse": lambda: lse_leg(torch, dev, stream, log),
                 "c4": lambda: c4_leg(torch, dev, stream, log),
-                "c5": lambda: c5_leg(log, graphs=not args.serve_eager)}
+                "c5": lambda: c5_leg(log, graphs=not args.serve_eager),
+                "bary": lambda: bary_leg(log)}
         print(json.dumps(legs[args.only]()), flush=True)
         return
 
@@ -464,6 +511,7 @@ def main():
         extra["lse_roofline"] = lse_leg(torch, dev, stream, log)
         extra["c4_wdl_step"] = c4_leg(torch, dev, stream, log)
         extra["c5_serving"] = c5_leg(log)
+        extra["barycenter_batched"] = bary_leg(log)
 
     res = {
         "metric": METRIC,
@@ -509,8 +557,9 @@ def main():
                                        "product, copy-calibrated "
                                        "(profiles/c2_gemm_hbm_traffic.md)",
                      "alg_bytes_per_launch": (4096 * 4096 + 1024 * 4096) * 2 + 4096 * 1024 * 4,
-                     "kernel": "k_gemm_tn<bf16> + k_gemm_splitk_epilogue (K^T W, 4096 x 1024 x "
-                               "4096, deferred epilogue, as in the MFMA path)",
+                     "kernel": "k_gemm_tn<bf16> + k_gemm_splitk_epilogue (4096 x 1024 x 4096, "
+                               "deferred epilogue, as the MFMA path runs Y = K X at 1024 "
+                               "targets)",
                      "kernel_ms": gemm_ms, "alg_flops_per_launch": gemm_flops},
         "cpu_baseline": cpu,
         **extra,

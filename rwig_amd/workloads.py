@@ -89,6 +89,21 @@ def c5_clouds(n_clouds=32, seed=1005):
     return out
 
 
+def bary_batch(n=2048, B=256, S=4, seed=1006):
+    """Batched barycenters on one cost: n points uniform in [0,1]^2, squared
+    Euclidean cost (as the serving clouds), B problems of S histograms
+    ~ Dirichlet(1) with weights ~ Dirichlet(1).
+    Returns (C (n x n, Fortran order), list of (A (n x S, Fortran order), w))."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 2))
+    sq = (X * X).sum(1)
+    C = np.maximum(sq[:, None] + sq[None, :] - 2.0 * X @ X.T, 0.0)
+    np.fill_diagonal(C, 0.0)
+    probs = [(np.asfortranarray(rng.dirichlet(np.ones(n), size=S).T), rng.dirichlet(np.ones(S)))
+             for _ in range(B)]
+    return np.asfortranarray(C), probs
+
+
 def c5_requests(sizes, count, seed=2005):
     """Serving mix, request side: 70% Sinkhorn (a, b ~ Dirichlet(1) on a
     random cloud), 30% barycenter (S in {2, 4, 8}, columns of A and the

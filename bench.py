@@ -77,7 +77,11 @@ def c2_gemm(torch, dev, stream):
 
 
 def gemm_only():
+    """The counter runs' workload: 5 BF16 products at the C2 shape, 5 copies of
+    256 MiB (the calibration of the byte counters), then 5 FP64 products of the
+    same shape (the absorption's exact marginals at 1024 targets)."""
     import torch
+    from rwig_amd._lib import check, lib
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     launch, _ = c2_gemm(torch, dev, stream)
@@ -87,9 +91,19 @@ def gemm_only():
         launch()
     for _ in range(5):
         dst.copy_(src)
+    gM, gN, gK = 4096, 1024, 4096
+    Ad = torch.rand((gM, gK), dtype=torch.float64, device=dev)
+    Bd = torch.rand((gN, gK), dtype=torch.float64, device=dev)
+    Dd = torch.empty((gN, gM), dtype=torch.float64, device=dev)
+    nb = ctypes.c_long()
+    check(lib().rwig_gemm_workspace_bytes(1, gM, gN, 1, ctypes.byref(nb)))
+    wk = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+    for _ in range(5):
+        check(lib().rwig_gemm_tn_f64_dev(gM, gN, gK, Ad.data_ptr(), gK, Bd.data_ptr(), gK,
+                                         Dd.data_ptr(), gM, 1, wk.data_ptr(), stream.cuda_stream))
     torch.cuda.synchronize()
-    print("gemm-only: 5 products 4096 x 1024 x 4096, 5 copies of 256 MiB",
-          flush=True)
+    print("gemm-only: 5 BF16 products 4096 x 1024 x 4096, 5 copies of 256 MiB, "
+          "5 FP64 products 4096 x 1024 x 4096", flush=True)
 
 
 def c3_leg(log):

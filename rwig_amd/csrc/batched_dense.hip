@@ -34,54 +34,6 @@ namespace rwig {
 
 namespace {
 
-// Tiles of the products by the number of active targets (the GEMM columns),
-// from sweeps at 4096 x targets x 4096 on MI355X: fewer columns take smaller
-// tiles, so that the product still spreads over the CUs. Each output element
-// sees the same sequence of MFMAs in every tile shape (no k split), so a
-// target's arithmetic does not depend on the tiles its column falls in.
-//   BF16 (us, with the deferred epilogue)      FP64 (us)
-//   targets <= 128: 32 x 32, 4 waves  16-20     <= 256: 32 x 32, 4 waves   100-210
-//           <= 256: 32 x 32, 1 wave   23-25     <= 512: 128 x 64, 8 waves  310-320
-//           <= 512: 64 x 64, 4 waves  31-34     <= 768: 64 x 64, 4 waves   495-510
-//           above: 128 x 128, 16 waves 44-50    above: 128 x 128, 16 waves 560-590
-// (the 128 x 128 tiles alone take 37-42 us in BF16 and 530-555 us in FP64 at
-// any count up to 512 targets).
-using Bf16Tiny = GemmShape<uint16_t, 32, 32, 64, 16, 2, 2, 3, 1>;
-using Bf16Small = GemmShape<uint16_t, 32, 32, 64, 16, 1, 1, 2, 1>;
-using Bf16Mid = GemmShape<uint16_t, 64, 64, 64, 16, 2, 2, 2, 1>;
-using F64Small = GemmShape<double, 32, 32, 16, 2, 2, 2, 3, 1>;
-using F64Mid = GemmShape<double, 128, 64, 16, 2, 4, 2, 3, 1>;
-
-template <class F> void with_bf16_tiles(int cols, F f) {
-  if (cols <= 128) f(Bf16Tiny{});
-  else if (cols <= 256) f(Bf16Small{});
-  else if (cols <= 512) f(Bf16Mid{});
-  else f(GemmCfg<uint16_t>{});
-}
-template <class F> void with_f64_tiles(int cols, F f) {
-  if (cols <= 256) f(F64Small{});
-  else if (cols <= 512) f(F64Mid{});
-  else if (cols <= 768) f(GemmCfg<double>{});
-  else f(GemmF64Wide{});
-}
-// largest split workspace (ks = 1) of a rows x cols product over every tile shape
-inline size_t bf16_work_elems(int rows, int cols) {
-  size_t n = 0;
-  for (int c : {1, 129, 257, 513})
-    with_bf16_tiles(c, [&](auto cfg) {
-      n = std::max(n, splitk_work_elems<uint16_t, decltype(cfg)>(rows, cols, 1));
-    });
-  return n;
-}
-inline size_t f64_work_elems(int rows, int cols) {
-  size_t n = 0;
-  for (int c : {1, 257, 513, 769})
-    with_f64_tiles(c, [&](auto cfg) {
-      n = std::max(n, splitk_work_elems<double, decltype(cfg)>(rows, cols, 1));
-    });
-  return n;
-}
-
 // per-target L1 (or reference) marginal error of the absorbed plan, stopping
 // decision, form of the next cycle; one block per target. cnt[0] counts the
 // targets stopped so far, cnt[1] the running ones in the multiplicative form.
@@ -260,17 +212,20 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   // slower (15.6k against 16.0k solves/s with the register-path tiles)
   SplitK<uint16_t> skb;
   skb.ks = 1;
-  skb.work = alloc_into<float>(pool, std::max(bf16_work_elems(M, B), bf16_work_elems(Nc, B)));
+  skb.work = alloc_into<float>(
+      pool, std::max(tiles_work_elems<uint16_t>(M, B, 1), tiles_work_elems<uint16_t>(Nc, B, 1)));
   SplitK<double> skd;
-  skd.work = alloc_into<double>(pool, std::max(f64_work_elems(M, B), f64_work_elems(Nc, B)));
+  skd.work = alloc_into<double>(
+      pool, std::max(tiles_work_elems<double>(M, B, 1), tiles_work_elems<double>(Nc, B, 1)));
 
   const uint16_t *Kb = dk.Kb.get(), *KbT = dk.KbT.get();
   const double *Kd = dk.Kd.get(), *KdT = dk.KdT.get();
   int nact = B;
-  // BF16 products over the active targets, in the tiles their count takes
+  // products over the active targets, in the tiles their count takes (the
+  // same arithmetic per target in every shape)
   auto bf16_product = [&](int rows, int kdim, const uint16_t *Am, long la, const uint16_t *Bm,
                           long lb, auto op) {
-    with_bf16_tiles(nact, [&](auto cfg) {
+    with_tiles<uint16_t>(nact, [&](auto cfg) {
       gemm_tn<uint16_t, decltype(cfg)>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr,
                                        skb);
     });
@@ -288,11 +243,11 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
                          skb.work, gx, ntile, nnz, tgt_of, col_idx, colpos, done, op);
       check_launch();
     };
-    with_bf16_tiles(nact, run);
+    with_tiles<uint16_t>(nact, run);
   };
   auto f64_product = [&](int rows, int kdim, const double *Am, long la, const double *Bm,
                          long lb, auto op) {
-    with_f64_tiles(nact, [&](auto cfg) {
+    with_tiles<double>(nact, [&](auto cfg) {
       gemm_tn<double, decltype(cfg)>(st, rows, nact, kdim, Am, la, Bm, lb, op, act, nullptr, skd);
     });
   };

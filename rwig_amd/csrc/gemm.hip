@@ -10,8 +10,8 @@ int rwig_gemm_workspace_bytes(int dtype, int M, int N, int ks, long *bytes) {
   return guarded("rwig_gemm_workspace_bytes", [&] {
     RWIG_REQUIRE(bytes && M > 0 && N > 0 && ks >= 1, "invalid argument");
     RWIG_REQUIRE(dtype == 0 || dtype == 1, "dtype must be 0 (bf16) or 1 (f64)");
-    *bytes = dtype == 0 ? (long)(4 * splitk_work_elems<uint16_t>(M, N, ks))
-                        : (long)(8 * splitk_work_elems<double>(M, N, ks));
+    *bytes = dtype == 0 ? (long)(4 * tiles_work_elems<uint16_t>(M, N, ks))
+                        : (long)(8 * tiles_work_elems<double>(M, N, ks));
   });
 }
 
@@ -22,10 +22,12 @@ int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda, const vo
     SplitK<uint16_t> sk;
     sk.ks = ks;
     sk.work = (float *)work;
-    gemm_tn<uint16_t>((hipStream_t)stream, M, N, K, (const uint16_t *)A, lda,
-                      (const uint16_t *)B, ldb,
-                      [=] __device__(int m, int n, float v) { D[m + (long)n * ldd] = v; },
-                      nullptr, nullptr, sk);
+    with_tiles<uint16_t>(N, [&](auto cfg) {
+      gemm_tn<uint16_t, decltype(cfg)>(
+          (hipStream_t)stream, M, N, K, (const uint16_t *)A, lda, (const uint16_t *)B, ldb,
+          [=] __device__(int m, int n, float v) { D[m + (long)n * ldd] = v; }, nullptr, nullptr,
+          sk);
+    });
   });
 }
 
@@ -36,9 +38,12 @@ int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda, const d
     SplitK<double> sk;
     sk.ks = ks;
     sk.work = (double *)work;
-    gemm_tn<double>((hipStream_t)stream, M, N, K, A, lda, B, ldb,
-                    [=] __device__(int m, int n, double v) { D[m + (long)n * ldd] = v; }, nullptr,
-                    nullptr, sk);
+    with_tiles<double>(N, [&](auto cfg) {
+      gemm_tn<double, decltype(cfg)>(
+          (hipStream_t)stream, M, N, K, A, lda, B, ldb,
+          [=] __device__(int m, int n, double v) { D[m + (long)n * ldd] = v; }, nullptr, nullptr,
+          sk);
+    });
   });
 }
 

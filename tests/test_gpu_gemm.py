@@ -125,3 +125,39 @@ def test_bf16_direct_to_lds_equals_register_path(M, N, K, ks, deferred):
                                       torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     assert torch.equal(Da, D)
+
+
+def run_f64(A, Bt, M, N, K, ks=1, deferred=False):
+    D = torch.empty((N, M), dtype=torch.float64, device="cuda")
+    w = workspace(1, M, N, ks) if deferred else None
+    check(lib().rwig_gemm_tn_f64_dev(M, N, K, A.data_ptr(), A.shape[1], Bt.data_ptr(),
+                                     Bt.shape[1], D.data_ptr(), M, ks,
+                                     None if w is None else w.data_ptr(),
+                                     torch.cuda.current_stream().cuda_stream))
+    return D
+
+
+@pytest.mark.parametrize("ks,deferred", [(1, False), (1, True), (3, True)])
+@pytest.mark.parametrize("dtype,K", [("bf16", 256), ("bf16", 200), ("f64", 256), ("f64", 201)])
+def test_every_tile_shape_gives_the_same_columns(dtype, K, ks, deferred):
+    """The tile shape follows the column count (smaller tiles for fewer
+    columns). A product over the first n columns must equal those columns of
+    the product over all of them bit for bit, for n in every shape's range.
+    K = 256 takes the direct-to-LDS tiles, K = 200 / 201 the register path."""
+    M = 300
+    g = torch.Generator(device="cpu").manual_seed(K + ks)
+    if dtype == "bf16":
+        Nall, subs, run = 700, (100, 200, 400), run_bf16
+        lda = (K + 7) // 8 * 8
+        A = torch.randn((M, lda), generator=g).to(torch.bfloat16).cuda()
+        Bt = torch.randn((Nall, lda), generator=g).to(torch.bfloat16).cuda()
+    else:
+        Nall, subs, run = 900, (200, 400, 700), run_f64
+        lda = (K + 1) // 2 * 2
+        A = torch.randn((M, lda), generator=g, dtype=torch.float64).cuda()
+        Bt = torch.randn((Nall, lda), generator=g, dtype=torch.float64).cuda()
+    full = run(A, Bt, M, Nall, K, ks, deferred)
+    for n in subs:
+        part = run(A, Bt[:n].contiguous(), M, n, K, ks, deferred)
+        torch.cuda.synchronize()
+        assert torch.equal(part, full[:n]), (dtype, K, n)

@@ -747,43 +747,19 @@ void transpose(hipStream_t st, const T *src, int rows, int cols, long lds, T *ds
 // R_s(i,j) = (C(i,j) - G(j,s)) - F(i,s) here (the row kernels' order of the
 // two subtractions), equal to the column kernels' up to rounding.
 
-// res[s] of column j from the chunk partials of a row pass over CT. A
-// workgroup covers 64 / SM columns: lane (column, slice) of wavefront w merges
-// chunks w, w + 8, ..., wavefront 0 combines the 8 states into LDS, then one
-// thread per column calls op with the column's S values.
+// Column form of a row pass over CT: v holds the merged value of slice s,
+// column j at [s * N + j] (k_lse_row_merge writes it, with coalesced loads of
+// the chunk partials), and op(j, res) gets column j's S values.
 template <typename T, int SM, class Op>
 __global__ void __launch_bounds__(kMergeTB)
-k_lse_col_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, int N, int S,
-                T reg, Op op, const LoopState *ls) {
+k_cols_apply(const T *__restrict__ v, int N, int S, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  constexpr int JB = kWave / SM;  // columns per workgroup
-  __shared__ T shm[kMergeWaves][kWave], sht[kMergeWaves][kWave];
-  __shared__ T r[JB][SM];
-  const T back = T((double)reg * kLn2);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int jj = lane / SM, s = lane % SM;
-  const int j = blockIdx.x * JB + jj;
-  const bool valid = j < N && s < S;
-  const long k = (long)s * N + j, len = (long)S * N;
-  SoftMin<T> a;
-  a.init();
-  if (valid)
-    for (int c = w; c < nchunk; c += kMergeWaves) a.merge(pm[c * len + k], ps[c * len + k]);
-  shm[w][lane] = a.m;
-  sht[w][lane] = a.t;
-  __syncthreads();
-  if (w == 0 && valid) {
-    SoftMin<T> b;
-    b.m = shm[0][lane];
-    b.t = sht[0][lane];
-    for (int u = 1; u < kMergeWaves; ++u) b.merge(shm[u][lane], sht[u][lane]);
-    r[jj][s] = b.value(back);
-  }
-  __syncthreads();
-  if (threadIdx.x < JB) {
-    const int jc = blockIdx.x * JB + threadIdx.x;
-    if (jc < N) op(jc, r[threadIdx.x]);
-  }
+  const int j = blockIdx.x * kMergeTB + threadIdx.x;
+  if (j >= N) return;
+  T res[SM];
+#pragma unroll
+  for (int s = 0; s < SM; ++s) res[s] = s < S ? v[(long)s * N + j] : T(0);
+  op(j, res);
 }
 
 template <typename T, int SM, class Op>
@@ -815,8 +791,9 @@ k_sum_cols(const T *__restrict__ part, int nchunk, int N, int S, Op op, const Lo
   }
 }
 
+// the chunk partials of a row pass over CT, then S x N for the merged values
 inline size_t col_t_part_size(int M, int N, int S) {
-  return (size_t)row_chunking(N, M).nchunk * S * N;
+  return ((size_t)row_chunking(N, M).nchunk + 1) * S * N;
 }
 
 // column soft-min of C (as lse_col) from CT; pm, ps: col_t_part_size each
@@ -830,8 +807,15 @@ void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F,
     hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg,
                        F, ldf, S, (T)reg, ch.cw, pm, ps, ls);
     check_launch();
-    hipLaunchKernelGGL((k_lse_col_merge<T, SM, Op>), dim3((N + kWave / SM - 1) / (kWave / SM)),
-                       dim3(kMergeTB), 0, st, pm, ps, ch.nchunk, N, S, (T)reg, op, ls);
+    // merged values into the room after the partials, then the column form
+    T *mv = pm + (size_t)ch.nchunk * S * N;
+    const long len = (long)S * N;
+    auto put = [=] __device__(long k, T r) { mv[k] = r; };
+    hipLaunchKernelGGL((k_lse_row_merge<T, decltype(put)>), dim3(merge_grid(len)), dim3(kMergeTB),
+                       0, st, pm, ps, ch.nchunk, len, (T)reg, put, ls);
+    check_launch();
+    hipLaunchKernelGGL((k_cols_apply<T, SM, Op>), dim3((N + kMergeTB - 1) / kMergeTB),
+                       dim3(kMergeTB), 0, st, mv, N, S, op, ls);
   });
   check_launch();
 }

@@ -7,8 +7,9 @@
 // alpha = exp(f/reg), beta = exp(g/reg); the absorbed plan of target t is
 // P0_ij = alpha_i K_ij beta_j and the current plan P = diag(u) P0 diag(v).
 //
-// Per iteration (BF16 MFMA, FP32 accumulation, Sinkhorn update fused into the
-// GEMM epilogue):
+// Per iteration (BF16 MFMA, FP32 accumulation; the Sinkhorn update reads the
+// product's FP32 tile: at the support entries only for Z, in the GEMM's
+// deferred epilogue for Y):
 //   Z = K^T W  ->  v on the supports, X = beta .* (v or v - 1) (zero off S_t)
 //   Y = K X    ->  u,                 W = alpha .* (u or u - 1)
 // Two forms of the update, chosen per target at every absorption:
@@ -25,7 +26,9 @@
 // v - 1 in the additive form); r0 = alpha .* (K beta), c0 = beta .* (K^T alpha) by FP64
 // MFMA GEMMs; the L1 marginal error of the absorbed plan decides convergence
 // (a target stops at the first absorption with error <= tol) and the form of
-// the next cycle.
+// the next cycle. Stopped targets leave the GEMM columns, and the products
+// take smaller tiles as the columns thin out. At the end one FP64 GEMM with
+// K o C gives each target's transport cost.
 #include "dense.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"

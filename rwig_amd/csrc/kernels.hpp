@@ -751,10 +751,10 @@ void transpose(hipStream_t st, const T *src, int rows, int cols, long lds, T *ds
 // column j at [s * N + j] (k_lse_row_merge writes it, with coalesced loads of
 // the chunk partials), and op(j, res) gets column j's S values.
 template <typename T, int SM, class Op>
-__global__ void __launch_bounds__(kMergeTB)
+__global__ void __launch_bounds__(kWave)
 k_cols_apply(const T *__restrict__ v, int N, int S, Op op, const LoopState *ls) {
   if (ls && !ls->active) return;
-  const int j = blockIdx.x * kMergeTB + threadIdx.x;
+  const int j = blockIdx.x * kWave + threadIdx.x;
   if (j >= N) return;
   T res[SM];
 #pragma unroll
@@ -814,8 +814,9 @@ void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F,
     hipLaunchKernelGGL((k_lse_row_merge<T, decltype(put)>), dim3(merge_grid(len)), dim3(kMergeTB),
                        0, st, pm, ps, ch.nchunk, len, (T)reg, put, ls);
     check_launch();
-    hipLaunchKernelGGL((k_cols_apply<T, SM, Op>), dim3((N + kMergeTB - 1) / kMergeTB),
-                       dim3(kMergeTB), 0, st, mv, N, S, op, ls);
+    // one wavefront per workgroup: N / 64 workgroups spread over the CUs
+    hipLaunchKernelGGL((k_cols_apply<T, SM, Op>), dim3((N + kWave - 1) / kWave), dim3(kWave), 0,
+                       st, mv, N, S, op, ls);
   });
   check_launch();
 }

@@ -202,3 +202,29 @@ def test_mfma_path_same_result_in_every_tile_shape(M):
         assert np.array_equal(one[0][0], mix[0][t]), t
         assert np.array_equal(one[1][0], mix[1][t]), t
         assert np.array_equal(one[4][0], mix[4][t]), t
+
+
+def test_mfma_path_at_the_full_c2_size():
+    """The headline workload at its full size: n = 4096, 1024 targets on 5%
+    supports, eps 0.01, absorption every 20, stop at L1 marginal error 1e-6.
+    Every target must report an error <= 1e-6; for 16 of them (spread over
+    the batch) the error, transport cost and regularised loss are recomputed
+    on the host from the returned potentials in FP64 and must agree."""
+    C, a, sup, wts, reg = workloads.c2()
+    tol = 1e-6
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    out = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, zero_tol=tol, max_iter=1000,
+                                        absorb_every=20), tb)
+    f, gs, it, err, loss = out
+    print("iterations min/mean/max", it.min(), it.mean(), it.max(), "max err", err.max(),
+          flush=True)
+    assert np.all(err <= tol)
+    assert np.all(it % 20 == 0) and it.max() < 1000
+    for t in np.linspace(0, len(sup) - 1, 16).astype(int):
+        e, c = plan_l1(C, a, sup[t], wts[t], f[t], gs[t], reg)
+        assert e <= tol * 1.0001, (t, e)
+        assert abs(err[t] - e) <= 1e-9 + 1e-6 * e, (t, err[t], e)
+        assert abs(loss[t, 0] - c) <= 1e-9 * abs(c), (t, loss[t, 0], c)
+        _, rl, mag = plan_loss(C, sup[t], f[t], gs[t], reg)
+        assert abs(loss[t, 1] - rl) <= 1e-12 * mag, (t, loss[t, 1], rl)

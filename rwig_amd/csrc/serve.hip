@@ -225,13 +225,16 @@ struct Group {
   int ks_y() const { return splitk_for<uint16_t>(M, N); }
   int ks_r() const { return splitk_for<double>(M, N); }
   int ks_c() const { return splitk_for<double>(N, M); }
+  // (the products take their tiles from the column capacity, with_tiles;
+  // the k split above does not depend on the tiles, so neither does a
+  // column's arithmetic)
   size_t wk_b_elems(int c) const {
-    return std::max(splitk_work_elems<uint16_t>(N, c, ks_z()),
-                    splitk_work_elems<uint16_t>(M, c, ks_y()));
+    return std::max(tiles_work_elems<uint16_t>(N, c, ks_z()),
+                    tiles_work_elems<uint16_t>(M, c, ks_y()));
   }
   size_t wk_d_elems(int c) const {
-    return std::max(splitk_work_elems<double>(M, c, ks_r()),
-                    splitk_work_elems<double>(N, c, ks_c()));
+    return std::max(tiles_work_elems<double>(M, c, ks_r()),
+                    tiles_work_elems<double>(N, c, ks_c()));
   }
 
   SplitK<uint16_t> skb(int ks) const {
@@ -370,6 +373,17 @@ struct Group {
     if (phase == 0) RWIG_HIP_CHECK(hipEventRecord(ev, st));
   }
 
+  // A product over the running columns (act, count from the device lists), in
+  // the tiles the column capacity takes.
+  template <typename T, class Op>
+  void product(hipStream_t st, int rows, int kdim, const T *A, long lda, const T *Bm, long ldb,
+               Op op, const SplitK<T> &sk) {
+    with_tiles<T>(cap, [&](auto cfg) {
+      gemm_tn<T, decltype(cfg)>(st, rows, cap, kdim, A, lda, Bm, ldb, op, d_act(), d_cnt() + kNact,
+                                sk);
+    });
+  }
+
   // Fresh state of the columns [c0, c0 + S) of a newly admitted request.
   void init_request(const Request &r) {
     hipStream_t st = stream->get();
@@ -457,14 +471,17 @@ struct Group {
       bep[q] = (float)bq;
       Bdp[j + t * ldni] = bq;
     });
-    gemm_tn<double>(
-        st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
-        [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
-        act, cnt + kNact, skd(ks_r()));
-    gemm_tn<double>(
-        st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
-        [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
-        act, cnt + kNact, skd(ks_c()));
+    with_tiles<double>(cp, [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<double, CF>(
+          st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
+          [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
+          act, cnt + kNact, skd(ks_r()));
+      gemm_tn<double, CF>(
+          st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
+          [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
+          act, cnt + kNact, skd(ks_c()));
+    });
     hipLaunchKernelGGL(k_serve_err, dim3(cp), dim3(256), 0, st, Mi, Ni, rg, d_areq(), d_riter(),
                        kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
                        tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,
@@ -479,19 +496,22 @@ struct Group {
       const long t = act[c];
       Xp[j + t * ldni] = f2bf(bep[j + t * Ni]);
     });
-    gemm_tn<uint16_t>(
-        st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
-        [=] __device__(int i, int t, float y) {
-          if (!dn[t]) Ycp[i + (long)t * Mi] = y;
-        },
-        act, cnt + kNact, skb(ks_y()));
-    gemm_tn<uint16_t>(
-        st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
-        [=] __device__(int j, int t, float z) {
-          const long q = j + (long)t * Ni;
-          if (!dn[t]) sbp[q] = (float)(c0p[q] / (double)z);
-        },
-        act, cnt + kNact, skb(ks_z()));
+    with_tiles<uint16_t>(cp, [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<uint16_t, CF>(
+          st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
+          [=] __device__(int i, int t, float y) {
+            if (!dn[t]) Ycp[i + (long)t * Mi] = y;
+          },
+          act, cnt + kNact, skb(ks_y()));
+      gemm_tn<uint16_t, CF>(
+          st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+          [=] __device__(int j, int t, float z) {
+            const long q = j + (long)t * Ni;
+            if (!dn[t]) sbp[q] = (float)(c0p[q] / (double)z);
+          },
+          act, cnt + kNact, skb(ks_z()));
+    });
     // u = a / r0 at v = 1; W for the first product of the cycle
     launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {
       const long t = act[c], q = i + t * Mi;
@@ -526,8 +546,8 @@ struct Group {
     float *lvp = lv.p<float>(), *lup = lu.p<float>(), *Zqp = Zq.p<float>();
     uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
     const DenseKernel &dk = gk->dk;
-    gemm_tn<uint16_t>(
-        st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+    product<uint16_t>(
+        st, Ni, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
         [=] __device__(int j, int t, float z) {
           const long q = j + (long)t * Ni;
           if (kd[t] == kBarycenter) {
@@ -546,7 +566,7 @@ struct Group {
             lvp[q] = (float)v;
           }
         },
-        act, cnt + kNact, skb(ks_z()));
+        skb(ks_z()));
     launch_cols(st, Ni, cp, cnt, kNbar, [=] __device__(int c, long j) {
       const long r = abar[c];
       const int S = np[r], m1 = md[r];
@@ -568,8 +588,8 @@ struct Group {
         lvp[q] = m1 ? (float)(v - 1.0) : (float)v;
       }
     });
-    gemm_tn<uint16_t>(
-        st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
+    product<uint16_t>(
+        st, Mi, Ni, dk.KbT.get(), ldni, Xp, ldni,
         [=] __device__(int i, int t, float y) {
           const long q = i + (long)t * Mi;
           const double a_ = (double)alp[q];
@@ -584,7 +604,7 @@ struct Group {
             lup[q] = (float)u;
           }
         },
-        act, cnt + kNact, skb(ks_y()));
+        skb(ks_y()));
   }
 
   // Copies of a finished request's results to the host (enqueued; complete

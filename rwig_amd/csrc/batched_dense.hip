@@ -116,9 +116,9 @@ k_dense_loss(int M, const int *__restrict__ col_ptr, const double *__restrict__ 
 // one thread per support entry p reads its own element, op(p, t, j, z). The
 // product's own epilogue would visit all Nc x targets entries, 95% of them
 // off the supports.
-template <class CF, class Op>
+template <class CF, class W, class Op>
 __global__ void __launch_bounds__(256)
-k_support(const float *__restrict__ work, int gx, long ntile, int nnz,
+k_support(const W *__restrict__ work, int gx, long ntile, int nnz,
           const int *__restrict__ tgt_of, const int *__restrict__ col_idx,
           const int *__restrict__ colpos, const int *__restrict__ done, Op op) {
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -198,7 +198,6 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   float *lv = alloc_into<float>(pool, nnz), *be = alloc_into<float>(pool, nnz);
   float *sb = alloc_into<float>(pool, nnz);
   double *c0 = alloc_into<double>(pool, nnz);
-  int *pos = alloc_into<int>(pool, (size_t)Nc * B);
   int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
   int *cnt = alloc_into<int>(pool, 2);  // targets stopped, multiplicative ones running
   int *act = alloc_into<int>(pool, B);  // targets still iterating (GEMM column map)
@@ -242,8 +241,8 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
                             [] __device__(int, int, float) {}, act, nullptr, skb, false);
       const int gx = (Nc + CF::BM - 1) / CF::BM;
       const long ntile = (long)gx * ((nact + CF::BN - 1) / CF::BN);
-      hipLaunchKernelGGL((k_support<CF, decltype(op)>), dim3((nnz + 255) / 256), dim3(256), 0, st,
-                         skb.work, gx, ntile, nnz, tgt_of, col_idx, colpos, done, op);
+      hipLaunchKernelGGL((k_support<CF, float, decltype(op)>), dim3((nnz + 255) / 256), dim3(256),
+                         0, st, skb.work, gx, ntile, nnz, tgt_of, col_idx, colpos, done, op);
       check_launch();
     };
     with_tiles<uint16_t>(nact, run);
@@ -263,10 +262,6 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   launch_map(st, (long)nnz, [=] __device__(long p) {
     G[p] = 0.0;
     lv[p] = 1.0f;
-  });
-  launch_map(st, (long)Nc * B, [=] __device__(long k) { pos[k] = -1; });
-  launch_map(st, (long)nnz, [=] __device__(long p) {
-    pos[col_idx[p] + (long)tgt_of[p] * Nc] = (int)p;
   });
   launch_map(st, (long)ldm * B, [=] __device__(long k) {
     W[k] = 0;
@@ -310,9 +305,17 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
     f64_product(M, Nc, KdT, ldn, Bd, ldn, [=] __device__(int i, int t, double v) {
       if (!done[t]) r0[i + (long)t * M] = v * Ad[i + (long)t * ldm];
     });
-    f64_product(Nc, M, Kd, ldm, Ad, ldm, [=] __device__(int j, int t, double v) {
-      const int p = pos[j + (long)t * Nc];
-      if (p >= 0 && !done[t]) c0[p] = v * Bd[j + (long)t * ldn];
+    // c0 on the supports only, from the product's partial tiles
+    with_tiles<double>(nact, [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<double, CF>(st, Nc, nact, M, Kd, ldm, Ad, ldm, [] __device__(int, int, double) {},
+                          act, nullptr, skd, false);
+      const int gx = (Nc + CF::BM - 1) / CF::BM;
+      const long ntile = (long)gx * ((nact + CF::BN - 1) / CF::BN);
+      auto op = [=] __device__(int p, int t, int j, double v) { c0[p] = v * Bd[j + (long)t * ldn]; };
+      hipLaunchKernelGGL((k_support<CF, double, decltype(op)>), dim3((nnz + 255) / 256), dim3(256),
+                         0, st, skd.work, gx, ntile, nnz, tgt_of, col_idx, colpos, done, op);
+      check_launch();
     });
     launch_map(st, 1L, [=] __device__(long) { cnt[1] = 0; });
     hipLaunchKernelGGL(k_dense_err, dim3(B), dim3(256), 0, st, M, B, a, r0, col_ptr, b, c0,

@@ -537,7 +537,7 @@ struct Group {
     const int Mi = M, Ni = N, cp = cap;
     const long ldmi = ldm, ldni = ldn;
     const double rg = reg;
-    const int *act = d_act(), *abar = d_abar(), *cnt = d_cnt();
+    const int *abar = d_abar(), *cnt = d_cnt();
     const int *kd = kind.p<int>(), *md = mode.p<int>(), *np = ns.p<int>();
     const double *c0p = c0.p<double>(), *r0p = r0.p<double>(), *ap = acol.p<double>(),
                  *bp = bcol.p<double>(), *Gp = G.p<double>(), *wp = wcol.p<double>();

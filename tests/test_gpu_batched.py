@@ -228,3 +228,45 @@ def test_mfma_path_at_the_full_c2_size():
         assert abs(loss[t, 0] - c) <= 1e-9 * abs(c), (t, loss[t, 0], c)
         _, rl, mag = plan_loss(C, sup[t], f[t], gs[t], reg)
         assert abs(loss[t, 1] - rl) <= 1e-12 * mag, (t, loss[t, 1], rl)
+
+
+def ref_err(C, a, sup, w, f, g, reg):
+    """The reference's error formula (src/sinkhorn_impl_log.cpp): L2 norms of
+    the log-marginal mismatches of the plan the potentials define."""
+    P = np.exp(-((C[:, sup] - f[:, None]) - g[None, :]) / reg)
+    return (np.linalg.norm(np.log(P.sum(1)) - np.log(a))
+            + np.linalg.norm(np.log(P.sum(0)) - np.log(w)))
+
+
+def test_mfma_path_edge_cases():
+    """Supports from 1 point to all M points; a cap on the iterations; the
+    reference error formula as the stopping test."""
+    C, a, sup, wts, reg = make(41, 64, 12, 1, 64, 0.1)
+    sup[0] = np.array([17])  # a one-point target: its plan is the source itself
+    wts[0] = np.array([1.0])
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    tol = 1e-6
+    out = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, zero_tol=tol, max_iter=2000), tb)
+    for t in range(tb.B):
+        e, _ = plan_l1(C, a, sup[t], wts[t], out[0][t], out[1][t], reg)
+        assert out[3][t] <= tol and e <= tol * 1.0001, (t, out[3][t], e)
+    P0 = np.exp(-((C[:, sup[0]] - out[0][0][:, None]) - out[1][0][None, :]) / reg)
+    assert np.abs(P0[:, 0] - a).max() <= 1e-6 * a.max()
+    # a cap: every target stops at max_iter (a multiple of the absorption
+    # interval) with the error of its absorbed plan
+    C2, a2, sup2, wts2, _ = make(42, 300, 5, 20, 60, 0.01)
+    tb2 = RB.TargetBatch(sup2, wts2)
+    Cd2, ad2 = to_dev(C2, a2)
+    capped = host(RB.sinkhorn_batched_mfma(Cd2, ad2, tb2, 0.01, zero_tol=1e-12, max_iter=40,
+                                           absorb_every=20), tb2)
+    assert np.all(capped[2] == 40)
+    for t in range(tb2.B):
+        e, _ = plan_l1(C2, a2, sup2[t], wts2[t], capped[0][t], capped[1][t], 0.01)
+        assert e > 1e-12 and abs(capped[3][t] - e) <= 1e-9 + 1e-6 * e, (t, capped[3][t], e)
+    # the reference formula decides the stop and is what is reported
+    refm = host(RB.sinkhorn_batched_mfma(Cd2, ad2, tb2, 0.01, zero_tol=tol, max_iter=3000,
+                                         err="reference"), tb2)
+    for t in range(tb2.B):
+        e = ref_err(C2, a2, sup2[t], wts2[t], refm[0][t], refm[1][t], 0.01)
+        assert refm[3][t] <= tol and abs(refm[3][t] - e) <= 1e-9 + 1e-6 * e, (t, refm[3][t], e)

@@ -974,8 +974,25 @@ class ServeEngine {
       g->fill_lists(0);
       g->run_phase(0);
     }
-    for (Group *g : active) {
-      RWIG_HIP_CHECK(hipEventSynchronize(g->ev));
+    // harvest the groups in the order their absorptions finish (polled), so
+    // that a group of large problems does not hold back the iterations of the
+    // groups behind it in the list; the next cycle starts once every group
+    // has been harvested
+    std::vector<Group *> waiting = active;
+    while (!waiting.empty()) {
+      Group *g = nullptr;
+      for (auto it = waiting.begin(); it != waiting.end(); ++it) {
+        const hipError_t q = hipEventQuery((*it)->ev);
+        if (q == hipErrorNotReady) continue;
+        RWIG_HIP_CHECK(q);
+        g = *it;
+        waiting.erase(it);
+        break;
+      }
+      if (!g) {
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
+        continue;
+      }
       std::vector<Request *> keep, fin;
       for (Request *r : g->running) {
         if (g->hdone_pin[r->col]) {

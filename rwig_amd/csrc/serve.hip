@@ -225,16 +225,16 @@ struct Group {
   int ks_y() const { return splitk_for<uint16_t>(M, N); }
   int ks_r() const { return splitk_for<double>(M, N); }
   int ks_c() const { return splitk_for<double>(N, M); }
-  // (the products take their tiles from the column capacity, with_tiles;
-  // the k split above does not depend on the tiles, so neither does a
-  // column's arithmetic)
+  // (the BF16 products take their tiles from the column capacity,
+  // with_tiles; the k split above does not depend on the tiles, so neither
+  // does a column's arithmetic)
   size_t wk_b_elems(int c) const {
     return std::max(tiles_work_elems<uint16_t>(N, c, ks_z()),
                     tiles_work_elems<uint16_t>(M, c, ks_y()));
   }
   size_t wk_d_elems(int c) const {
-    return std::max(tiles_work_elems<double>(M, c, ks_r()),
-                    tiles_work_elems<double>(N, c, ks_c()));
+    return std::max(splitk_work_elems<double>(M, c, ks_r()),
+                    splitk_work_elems<double>(N, c, ks_c()));
   }
 
   SplitK<uint16_t> skb(int ks) const {
@@ -471,17 +471,16 @@ struct Group {
       bep[q] = (float)bq;
       Bdp[j + t * ldni] = bq;
     });
-    with_tiles<double>(cp, [&](auto cfg) {
-      using CF = decltype(cfg);
-      gemm_tn<double, CF>(
-          st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
-          [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
-          act, cnt + kNact, skd(ks_r()));
-      gemm_tn<double, CF>(
-          st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
-          [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
-          act, cnt + kNact, skd(ks_c()));
-    });
+    // (the FP64 products keep the default tiles: with their k split and the
+    // register path that ragged n takes, smaller FP64 tiles measured slower)
+    gemm_tn<double>(
+        st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
+        [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
+        act, cnt + kNact, skd(ks_r()));
+    gemm_tn<double>(
+        st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
+        [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
+        act, cnt + kNact, skd(ks_c()));
     hipLaunchKernelGGL(k_serve_err, dim3(cp), dim3(256), 0, st, Mi, Ni, rg, d_areq(), d_riter(),
                        kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
                        tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,

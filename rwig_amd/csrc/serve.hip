@@ -79,6 +79,11 @@ constexpr int kNact = 0, kNreq = 1, kNbar = 2;
 // columns; for a barycenter the common target is re-estimated from the exact
 // column marginals, b = prod_s (c0_s e^{-g_s/reg})^{w_s}, and stored in the
 // request's first target column.
+//
+// A request's rows and columns are split over gridDim.y blocks: each writes
+// its partial sum to part[base * gridDim.y + y], and the block that arrives
+// last (a counter per request, reset by that block) adds the partials in
+// block order, so the error does not depend on which block finishes last.
 __global__ void __launch_bounds__(256)
 k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
             const int *__restrict__ riter, const int *__restrict__ kind,
@@ -86,21 +91,25 @@ k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
             const int *__restrict__ maxit, const double *__restrict__ tol,
             const double *__restrict__ acol, const double *__restrict__ r0, double *bcol,
             const double *__restrict__ c0, const double *__restrict__ G, double switch_thr,
-            int *done, int *mode, double *errs, const int *cnt) {
+            int *done, int *mode, double *errs, const int *cnt, double *part, int *arrive) {
   __shared__ double sh[256 / kWave];
+  __shared__ int last;
   if ((int)blockIdx.x >= cnt[kNreq]) return;
   const int base = areq[blockIdx.x];
   const int S = ns[base];
+  const int nb = gridDim.y, y = blockIdx.y;
+  const int cm = (M + nb - 1) / nb, cn = (N + nb - 1) / nb;
+  const int i0 = y * cm, i1 = min(M, i0 + cm), j0 = y * cn, j1 = min(N, j0 + cn);
   double e = 0.0;
   for (int s = 0; s < S; ++s) {
     const long o = (long)(base + s) * M;
-    for (int i = threadIdx.x; i < M; i += blockDim.x) e += fabs(r0[o + i] - acol[o + i]);
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) e += fabs(r0[o + i] - acol[o + i]);
   }
   if (kind[base] == kSinkhorn) {
     const long o = (long)base * N;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) e += fabs(c0[o + j] - bcol[o + j]);
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) e += fabs(c0[o + j] - bcol[o + j]);
   } else {
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
       double lb = 0.0;
       for (int s = 0; s < S; ++s) {
         const long q = j + (long)(base + s) * N;
@@ -113,6 +122,18 @@ k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
   }
   e = block_sum(e, sh);
   if (threadIdx.x == 0) {
+    part[(long)base * nb + y] = e;
+    __threadfence();
+    last = atomicAdd(&arrive[base], 1) == nb - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const volatile double *pv = part + (long)base * nb;
+    e = 0.0;
+    for (int q = 0; q < nb; ++q) e += pv[q];
+    arrive[base] = 0;
     const int it = riter[blockIdx.x];
     errs[base] = e;
     const int stop = (it < maxit[base] && e >= tol[base]) ? 0 : 1;
@@ -187,6 +208,10 @@ struct Group {
   Slab wcol, tol, errs;                      // doubles per column
   // device lists: header (nact, nreq, nbar, -) | act | areq | riter | abar
   DBuf<int> lists;
+  // stopping test spread over err_blocks() blocks per request: partial sums
+  // and per-request arrival counters (zero between launches)
+  DBuf<double> errpart;
+  DBuf<int> arrive;
   int *hl[2] = {nullptr, nullptr};  // pinned host copies, one per cycle phase
   int *hdone_pin = nullptr;
   double *herr_pin = nullptr;
@@ -250,9 +275,12 @@ struct Group {
     return k;
   }
 
+  int err_blocks() const { return std::min(16, std::max(1, (std::max(M, N) + 1023) / 1024)); }
+
   // Device bytes of the group at c columns (layout() set).
   size_t bytes(int c) {
-    return per_col_bytes() * (size_t)c + 4 * wk_b_elems(c) + 8 * wk_d_elems(c);
+    return per_col_bytes() * (size_t)c + 4 * wk_b_elems(c) + 8 * wk_d_elems(c) +
+           (8 * (size_t)err_blocks() + 4) * c;
   }
 
   void drop_graphs() {
@@ -285,6 +313,10 @@ struct Group {
       s->buf = std::move(nb);
     }
     lists.reset(4 + 4 * (size_t)ncap);
+    errpart.reset((size_t)ncap * err_blocks());
+    arrive.reset(ncap);
+    int *ar = arrive.get();
+    launch_map(stream->get(), (long)ncap, [=] __device__(long k) { ar[k] = 0; });
     wk_b.reset(wk_b_elems(ncap));
     wk_d.reset(wk_d_elems(ncap));
     for (int *&h : hl) {
@@ -481,10 +513,10 @@ struct Group {
         st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
         [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
         act, cnt + kNact, skd(ks_c()));
-    hipLaunchKernelGGL(k_serve_err, dim3(cp), dim3(256), 0, st, Mi, Ni, rg, d_areq(), d_riter(),
-                       kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
+    hipLaunchKernelGGL(k_serve_err, dim3(cp, err_blocks()), dim3(256), 0, st, Mi, Ni, rg, d_areq(),
+                       d_riter(), kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
                        tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,
-                       errs.p<double>(), cnt);
+                       errs.p<double>(), cnt, errpart.get(), arrive.get());
     check_launch();
     // calibration: the same BF16 products at u = v = 1
     launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {

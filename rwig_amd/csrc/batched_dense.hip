@@ -130,6 +130,44 @@ k_support(const W *__restrict__ work, int gx, long ntile, int nnz,
   op(p, t, j, work[tile_partial<CF>(j, n, 0, gx, ntile)]);
 }
 
+// Update over the rows from the partial tiles of a product whose rows are the
+// M rows of C and whose columns are the active targets (Y = K X), for M a
+// multiple of 4: one thread per 4 consecutive rows of one column, in the
+// order the tiles are stored (16-byte loads of the tile), op(i, t, y) for
+// rows i..i+3 of target act[n]. The product's own epilogue visits one
+// element per thread.
+template <class CF, class Op>
+__global__ void __launch_bounds__(256)
+k_rows4(const float *__restrict__ work, int M, int nact, int gx, long nq,
+        const int *__restrict__ act, Op op) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nq) return;
+  constexpr int QT = CF::BM * CF::BN / 4;  // groups of 4 per tile
+  const long tile = q / QT;
+  const int pos = (int)(q % QT);
+  const int nl = pos / (CF::BM / 4), ml = (pos % (CF::BM / 4)) * 4;
+  const int m = (int)(tile % gx) * CF::BM + ml, n = (int)(tile / gx) * CF::BN + nl;
+  if (m >= M || n >= nact) return;
+  op(m, act[n], reinterpret_cast<const float4 *>(work)[q]);
+}
+
+// The Y = K X update of one element: u from the product y, then W (the next
+// product's operand) and u as kept between absorptions (u - 1 in the
+// additive form).
+__device__ inline void ky_update(bool add, double ai, double r0k, float alk, float rak, float y,
+                                 uint16_t &w, float &luk) {
+  double u;
+  if (add) {
+    u = ai / (r0k + (double)alk * (double)y);
+    w = f2bf((float)((double)alk * (u - 1.0)));
+    luk = (float)(u - 1.0);
+  } else {
+    u = ai / ((double)rak * (double)y);
+    w = f2bf((float)((double)alk * u));
+    luk = (float)u;
+  }
+}
+
 template <typename T> T *alloc_into(std::vector<DBuf<char>> &pool, size_t n) {
   pool.emplace_back(n * sizeof(T));
   return reinterpret_cast<T *>(pool.back().get());
@@ -370,19 +408,55 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
       lv[p] = mode[t] ? (float)(v - 1.0) : (float)v;
     });
     // Y = K X -> u, W
-    bf16_product(M, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
-      if (done[t]) return;
-      const long k = i + (long)t * M;
-      double u;
-      if (mode[t]) {
-        u = a[i] / (r0[k] + (double)al[k] * (double)y);
-        W[i + (long)t * ldm] = f2bf((float)((double)al[k] * (u - 1.0)));
-      } else {
-        u = a[i] / ((double)ra[k] * (double)y);
-        W[i + (long)t * ldm] = f2bf((float)((double)al[k] * u));
-      }
-      if (keep) lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
-    });
+    if (M % 4 == 0) {
+      // 4 rows per thread from the product's partial tiles (16-byte loads
+      // of y, al, ra and r0, 8-byte stores of W)
+      auto op = [=] __device__(int i, int t, float4 y4) {
+        if (done[t]) return;
+        const long k = i + (long)t * M;
+        const bool add = mode[t] != 0;
+        const float4 al4 = *reinterpret_cast<const float4 *>(al + k);
+        const float yv[4] = {y4.x, y4.y, y4.z, y4.w}, alv[4] = {al4.x, al4.y, al4.z, al4.w};
+        double r0v[4] = {0.0, 0.0, 0.0, 0.0};
+        float rav[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (add) {
+          const double2 p = *reinterpret_cast<const double2 *>(r0 + k),
+                        q = *reinterpret_cast<const double2 *>(r0 + k + 2);
+          r0v[0] = p.x, r0v[1] = p.y, r0v[2] = q.x, r0v[3] = q.y;
+        } else {
+          const float4 ra4 = *reinterpret_cast<const float4 *>(ra + k);
+          rav[0] = ra4.x, rav[1] = ra4.y, rav[2] = ra4.z, rav[3] = ra4.w;
+        }
+        uint16_t w[4];
+        float l[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ky_update(add, a[i + r], r0v[r], alv[r], rav[r], yv[r], w[r], l[r]);
+        *reinterpret_cast<uint2 *>(W + i + (long)t * ldm) =
+            make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
+        if (keep) *reinterpret_cast<float4 *>(lu + k) = make_float4(l[0], l[1], l[2], l[3]);
+      };
+      with_tiles<uint16_t>(nact, [&](auto cfg) {
+        using CF = decltype(cfg);
+        gemm_tn<uint16_t, CF>(st, M, nact, Nc, KbT, ldn, X, ldn,
+                              [] __device__(int, int, float) {}, act, nullptr, skb, false);
+        const int gx = (M + CF::BM - 1) / CF::BM;
+        const long nq = (long)gx * ((nact + CF::BN - 1) / CF::BN) * (CF::BM * CF::BN / 4);
+        hipLaunchKernelGGL((k_rows4<CF, decltype(op)>), dim3((unsigned)((nq + 255) / 256)),
+                           dim3(256), 0, st, skb.work, M, nact, gx, nq, act, op);
+        check_launch();
+      });
+    } else {
+      bf16_product(M, Nc, KbT, ldn, X, ldn, [=] __device__(int i, int t, float y) {
+        if (done[t]) return;
+        const long k = i + (long)t * M;
+        const bool add = mode[t] != 0;
+        uint16_t w;
+        float l;
+        ky_update(add, a[i], add ? r0[k] : 0.0, al[k], add ? 0.0f : ra[k], y, w, l);
+        W[i + (long)t * ldm] = w;
+        if (keep) lu[k] = l;
+      });
+    }
   };
 
   // active list: every target at first; after each absorption the targets

@@ -246,12 +246,14 @@ inline size_t splitk_work_elems(int M, int N, int ks) {
 // sequence of MFMAs in every tile shape (for a given k split), so a column's
 // arithmetic does not depend on the tiles it falls in.
 //   BF16 (us, with the deferred epilogue)      FP64 (us)
-//   cols <= 128: 32 x 32, 4 waves     16-20     <= 256: 32 x 32, 4 waves   100-210
+//   cols <=  32: 16 x 32, 2 waves     15.3-15.5
+//        <= 128: 32 x 32, 4 waves     16-20     <= 256: 32 x 32, 4 waves   100-210
 //        <= 256: 32 x 32, 1 wave      23-25     <= 512: 128 x 64, 8 waves  310-320
 //        <= 512: 64 x 64, 4 waves     31-34     <= 768: 64 x 64, 4 waves   495-510
 //        above: 128 x 128, 16 waves   44-50     above: 128 x 128, 16 waves 560-590
 // (the 128 x 128 tiles alone take 37-42 us in BF16 and 530-555 us in FP64 at
 // any count up to 512 columns). FP32 products keep GemmCfg<float>.
+using Bf16Micro = GemmShape<uint16_t, 16, 32, 64, 16, 1, 2, 3, 1>;
 using Bf16Tiny = GemmShape<uint16_t, 32, 32, 64, 16, 2, 2, 3, 1>;
 using Bf16Small = GemmShape<uint16_t, 32, 32, 64, 16, 1, 1, 2, 1>;
 using Bf16Mid = GemmShape<uint16_t, 64, 64, 64, 16, 2, 2, 2, 1>;
@@ -260,7 +262,8 @@ using F64Mid = GemmShape<double, 128, 64, 16, 2, 4, 2, 3, 1>;
 
 template <typename T, class F> void with_tiles(int cols, F f) {
   if constexpr (std::is_same<T, uint16_t>::value) {
-    if (cols <= 128) f(Bf16Tiny{});
+    if (cols <= 32) f(Bf16Micro{});
+    else if (cols <= 128) f(Bf16Tiny{});
     else if (cols <= 256) f(Bf16Small{});
     else if (cols <= 512) f(Bf16Mid{});
     else f(GemmCfg<uint16_t>{});
@@ -278,7 +281,7 @@ template <typename T, class F> void with_tiles(int cols, F f) {
 // tile shapes with_tiles picks
 template <typename T> inline size_t tiles_work_elems(int rows, int cols, int ks) {
   size_t n = 0;
-  for (int c : {1, 129, 257, 513, 769})
+  for (int c : {1, 33, 129, 257, 513, 769})
     with_tiles<T>(c, [&](auto cfg) {
       n = std::max(n, splitk_work_elems<T, decltype(cfg)>(rows, cols, ks));
     });

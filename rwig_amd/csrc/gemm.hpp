@@ -96,7 +96,7 @@ template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, i
           int DMA_ = 0>
 struct GemmShape {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WGM = WGM_, WGN = WGN_;
-  static constexpr int PF = PF_;  // k-steps of global loads in flight (1 to 4)
+  static constexpr int PF = PF_;  // k-steps of global loads in flight (>= 1)
   static constexpr int DMA = DMA_;
   static constexpr int NT = 64 * WGM * WGN;  // threads per workgroup
   static constexpr int WM = BM / WGM, WN = BN / WGN;
@@ -330,6 +330,17 @@ template <int V> __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((V & 15) | (7 << 4) | (15 << 8) | ((V >> 4) << 14));
 }
 
+// Waits until at most `later` k-steps of MOVES moves each are outstanding
+// (later <= L, a compile-time count per branch for s_waitcnt).
+template <int MOVES, int L> __device__ __forceinline__ void wait_later(int later) {
+  if constexpr (L == 0) {
+    wait_vmcnt<0>();
+  } else {
+    if (later >= L) wait_vmcnt<L * MOVES>();
+    else wait_later<MOVES, L - 1>(later);
+  }
+}
+
 // Main loop, register path: global loads into PF register sets, stored into
 // one of two padded LDS stages after the MFMA work of the current k-step.
 template <typename T, class CF>
@@ -441,6 +452,7 @@ __device__ __forceinline__ void gemm_main_dma(
   constexpr int STAGEB = (BM + BN) * ROWB;
   constexpr int MA = BM / NW / RPM, MB = BN / NW / RPM;  // moves per wavefront and k-step
   constexpr int MOVES = MA + MB;
+  static_assert((PF - 1) * MOVES < 64, "the moves in flight must fit vmcnt");
   __shared__ __attribute__((aligned(1024))) char smem[STAGES * STAGEB];
   RWIG_LDS char *lds = (RWIG_LDS char *)smem;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -485,21 +497,7 @@ __device__ __forceinline__ void gemm_main_dma(
     // this wavefront's moves of step s have landed once only the later
     // steps' moves are outstanding
     const int later = min(nst, s + PF) - (s + 1);
-    if constexpr (PF >= 4) {
-      if (later >= 3) wait_vmcnt<3 * MOVES>();
-      else if (later == 2) wait_vmcnt<2 * MOVES>();
-      else if (later == 1) wait_vmcnt<MOVES>();
-      else wait_vmcnt<0>();
-    } else if constexpr (PF == 3) {
-      if (later >= 2) wait_vmcnt<2 * MOVES>();
-      else if (later == 1) wait_vmcnt<MOVES>();
-      else wait_vmcnt<0>();
-    } else if constexpr (PF == 2) {
-      if (later >= 1) wait_vmcnt<MOVES>();
-      else wait_vmcnt<0>();
-    } else {
-      wait_vmcnt<0>();
-    }
+    wait_later<MOVES, PF - 1>(later);
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");

@@ -588,26 +588,13 @@ __host__ __device__ inline long tile_partial(int m, int n, int z, int gx, long n
   return ((long)z * ntile + tile) * CF::BM * CF::BN + (long)(n % CF::BN) * CF::BM + m % CF::BM;
 }
 
-// Each thread takes V consecutive rows of one column (V = 1, or 16 bytes of
-// partials: 4 in FP32, 2 in FP64) and adds the splits in split order, so the
-// sums do not depend on V.
-template <typename T> constexpr int epi_vec() {
-  return 16 / (int)sizeof(typename Mfma<T>::out_t);
-}
-template <class CF, int V> constexpr int epi_threads() {
-  return CF::BM * kEpiCols / V < 256 ? CF::BM * kEpiCols / V : 256;
-}
-
-template <typename T, class CF, int V, class Op>
+template <typename T, class CF, class Op>
 __global__ void __launch_bounds__(256)
 k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
                        const int *__restrict__ ncount, int ks,
                        const typename Mfma<T>::out_t *__restrict__ work, Op op) {
-  using O = typename Mfma<T>::out_t;
-  struct alignas(sizeof(O) * V) Vec { O v[V]; };
   constexpr int BM = CF::BM, BN = CF::BN;
-  constexpr int NT = epi_threads<CF, V>(), G = BM * kEpiCols / V;
-  static_assert(BM % V == 0 && G % NT == 0 && BN % kEpiCols == 0, "epilogue tiling");
+  static_assert((BM * kEpiCols) % 256 == 0 && BN % kEpiCols == 0, "epilogue tiling");
   if (ncount) {
     const int nc = *ncount;
     if ((int)blockIdx.y * BN >= nc) return;
@@ -616,25 +603,18 @@ k_gemm_splitk_epilogue(int M, int N, const int *__restrict__ nmap,
   const long ntile = (long)gridDim.x * gridDim.y;
   const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
   const long zs = ntile * BM * BN;
-  const O *w0 = work + tile * BM * BN;
+  const typename Mfma<T>::out_t *w0 = work + tile * BM * BN;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 #pragma unroll
-  for (int pass = 0; pass < G / NT; ++pass) {
-    const int idx = pass * NT + threadIdx.x;
-    const int nl = blockIdx.z * kEpiCols + idx / (BM / V), ml = (idx % (BM / V)) * V;
+  for (int pass = 0; pass < BM * kEpiCols / 256; ++pass) {
+    const int idx = pass * 256 + threadIdx.x;
+    const int nl = blockIdx.z * kEpiCols + idx / BM, ml = idx % BM;
     const int m = m0 + ml, n = n0 + nl;
     if (m >= M || n >= N) continue;
     const long e = (long)nl * BM + ml;
-    Vec v = *reinterpret_cast<const Vec *>(w0 + e);
-    for (int z = 1; z < ks; ++z) {
-      const Vec p = *reinterpret_cast<const Vec *>(w0 + z * zs + e);
-#pragma unroll
-      for (int r = 0; r < V; ++r) v.v[r] += p.v[r];
-    }
-    const int c = nmap ? nmap[n] : n;
-#pragma unroll
-    for (int r = 0; r < V; ++r)
-      if (m + r < M) op(m + r, c, v.v[r]);
+    typename Mfma<T>::out_t v = w0[e];
+    for (int z = 1; z < ks; ++z) v += w0[z * zs + e];
+    op(m, nmap ? nmap[n] : n, v);
   }
 }
 
@@ -676,16 +656,8 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
                      ldb, nmap, ncount, sk, op);
   check_launch();
   if (sk.work && epilogue) {
-    const dim3 eg(grid.x, grid.y, BN / kEpiCols);
-    if ((reinterpret_cast<uintptr_t>(sk.work) & 15) == 0) {
-      constexpr int V = epi_vec<T>();
-      hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, CF, V, Op>), eg,
-                         dim3(epi_threads<CF, V>()), 0, st, M, N, nmap, ncount, sk.ks, sk.work,
-                         op);
-    } else {
-      hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, CF, 1, Op>), eg, dim3(epi_threads<CF, 1>()),
-                         0, st, M, N, nmap, ncount, sk.ks, sk.work, op);
-    }
+    hipLaunchKernelGGL((k_gemm_splitk_epilogue<T, CF, Op>), dim3(grid.x, grid.y, BN / kEpiCols),
+                       dim3(256), 0, st, M, N, nmap, ncount, sk.ks, sk.work, op);
     check_launch();
   }
 }

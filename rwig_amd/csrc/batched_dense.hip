@@ -151,20 +151,24 @@ k_rows4(const float *__restrict__ work, int M, int nact, int gx, long nq,
   op(m, act[n], reinterpret_cast<const float4 *>(work)[q]);
 }
 
-// The Y = K X update of one element: u from the product y, then W (the next
-// product's operand) and u as kept between absorptions (u - 1 in the
-// additive form).
-__device__ inline void ky_update(bool add, double ai, double r0k, float alk, float rak, float y,
+// The Y = K X update of one element from the product y and the cycle's
+// factor q of the element: W (the next product's operand) and, when kept, u
+// (u - 1 in the additive form).
+//   multiplicative: q = alpha / ra, W = alpha u = a q / y, u = a q / (alpha y)
+//   additive:       q = a - r0 (FP32: the defect, small near convergence),
+//                   u = a / ((a - q) + alpha y), W = alpha (u - 1)
+// alpha is read by the additive form and, in the multiplicative one, only for
+// the kept u.
+__device__ inline void ky_update(bool add, bool keep, double ai, float qk, float alk, float y,
                                  uint16_t &w, float &luk) {
-  double u;
   if (add) {
-    u = ai / (r0k + (double)alk * (double)y);
+    const double u = ai / ((ai - (double)qk) + (double)alk * (double)y);
     w = f2bf((float)((double)alk * (u - 1.0)));
     luk = (float)(u - 1.0);
   } else {
-    u = ai / ((double)rak * (double)y);
-    w = f2bf((float)((double)alk * u));
-    luk = (float)u;
+    const double wy = ai * (double)qk / (double)y;
+    w = f2bf((float)wy);
+    if (keep) luk = (float)(wy / (double)alk);
   }
 }
 
@@ -231,7 +235,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   std::vector<DBuf<char>> pool;
   // per-target state
   float *lu = alloc_into<float>(pool, MB), *al = alloc_into<float>(pool, MB);
-  float *ra = alloc_into<float>(pool, MB);
+  float *qy = alloc_into<float>(pool, MB);  // the Y = K X update's factor (ky_update)
   double *r0 = alloc_into<double>(pool, MB);
   float *lv = alloc_into<float>(pool, nnz), *be = alloc_into<float>(pool, nnz);
   float *sb = alloc_into<float>(pool, nnz);
@@ -381,7 +385,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
     launch_map(st, MB, [=] __device__(long k) {
       const long t = k / M, i = k % M;
       if (done[t]) return;
-      if (!mode[t]) ra[k] = (float)(r0[k] / (double)Yc[k]);
+      qy[k] = mode[t] ? (float)(a[i] - r0[k]) : (float)((double)al[k] * (double)Yc[k] / r0[k]);
       const double u = a[i] / r0[k];
       lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
       W[i + t * ldm] = f2bf(mode[t] ? (float)((double)al[k] * (u - 1.0)) : (float)((double)al[k] * u));
@@ -410,27 +414,22 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
     // Y = K X -> u, W
     if (M % 4 == 0) {
       // 4 rows per thread from the product's partial tiles (16-byte loads
-      // of y, al, ra and r0, 8-byte stores of W)
+      // of y, q and alpha, 8-byte stores of W)
       auto op = [=] __device__(int i, int t, float4 y4) {
         if (done[t]) return;
         const long k = i + (long)t * M;
         const bool add = mode[t] != 0;
-        const float4 al4 = *reinterpret_cast<const float4 *>(al + k);
-        const float yv[4] = {y4.x, y4.y, y4.z, y4.w}, alv[4] = {al4.x, al4.y, al4.z, al4.w};
-        double r0v[4] = {0.0, 0.0, 0.0, 0.0};
-        float rav[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (add) {
-          const double2 p = *reinterpret_cast<const double2 *>(r0 + k),
-                        q = *reinterpret_cast<const double2 *>(r0 + k + 2);
-          r0v[0] = p.x, r0v[1] = p.y, r0v[2] = q.x, r0v[3] = q.y;
-        } else {
-          const float4 ra4 = *reinterpret_cast<const float4 *>(ra + k);
-          rav[0] = ra4.x, rav[1] = ra4.y, rav[2] = ra4.z, rav[3] = ra4.w;
+        const float4 q4 = *reinterpret_cast<const float4 *>(qy + k);
+        const float yv[4] = {y4.x, y4.y, y4.z, y4.w}, qv[4] = {q4.x, q4.y, q4.z, q4.w};
+        float alv[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        if (add || keep) {
+          const float4 al4 = *reinterpret_cast<const float4 *>(al + k);
+          alv[0] = al4.x, alv[1] = al4.y, alv[2] = al4.z, alv[3] = al4.w;
         }
         uint16_t w[4];
         float l[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) ky_update(add, a[i + r], r0v[r], alv[r], rav[r], yv[r], w[r], l[r]);
+        for (int r = 0; r < 4; ++r) ky_update(add, keep, a[i + r], qv[r], alv[r], yv[r], w[r], l[r]);
         *reinterpret_cast<uint2 *>(W + i + (long)t * ldm) =
             make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
         if (keep) *reinterpret_cast<float4 *>(lu + k) = make_float4(l[0], l[1], l[2], l[3]);
@@ -452,7 +451,7 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
         const bool add = mode[t] != 0;
         uint16_t w;
         float l;
-        ky_update(add, a[i], add ? r0[k] : 0.0, al[k], add ? 0.0f : ra[k], y, w, l);
+        ky_update(add, keep, a[i], qy[k], (add || keep) ? al[k] : 1.0f, y, w, l);
         W[i + (long)t * ldm] = w;
         if (keep) lu[k] = l;
       });

@@ -21,6 +21,10 @@
 //     with W = alpha (u - 1), X = beta (v - 1): the exact FP64 marginals r0, c0
 //     carry the O(1) part and the BF16 products only the corrections, whose
 //     rounding vanishes with (u - 1, v - 1) -- the fixed point is the FP64 one.
+// Each cycle folds an update's per-element inputs into one FP32 factor:
+// alpha / ra or the defect a - r0 for Y, beta b / sb or the defect b - c0 for
+// Z. Near convergence the defects are small, so their FP32 rounding moves the
+// fixed point by a relative 6e-8 of the defect.
 // Absorption (FP64, every absorb_every iterations): f += reg log u,
 // g += reg log v (between absorptions u, v are kept in FP32, as u - 1 and
 // v - 1 in the additive form); r0 = alpha .* (K beta), c0 = beta .* (K^T alpha) by FP64
@@ -128,6 +132,20 @@ k_support(const W *__restrict__ work, int gx, long ntile, int nnz,
   if (n < 0 || done[t]) return;
   const int j = col_idx[p];
   op(p, t, j, work[tile_partial<CF>(j, n, 0, gx, ntile)]);
+}
+
+// The K^T W update of a cycle: as k_support, with the entry's GEMM column and
+// form resolved once per cycle into wn[p] = (column << 1) | additive, or -1
+// for a stopped target; op(p, additive, z).
+template <class CF, class Op>
+__global__ void __launch_bounds__(256)
+k_support_cycle(const float *__restrict__ work, int gx, long ntile, int nnz,
+                const int *__restrict__ col_idx, const int *__restrict__ wn, Op op) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= nnz) return;
+  const int w = wn[p];
+  if (w < 0) return;
+  op(p, (w & 1) != 0, work[tile_partial<CF>(col_idx[p], w >> 1, 0, gx, ntile)]);
 }
 
 // Update over the rows from the partial tiles of a product whose rows are the
@@ -238,6 +256,8 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
   float *qy = alloc_into<float>(pool, MB);  // the Y = K X update's factor (ky_update)
   double *r0 = alloc_into<double>(pool, MB);
   float *lv = alloc_into<float>(pool, nnz), *be = alloc_into<float>(pool, nnz);
+  float *qz = alloc_into<float>(pool, nnz);  // the K^T W update's factor of a cycle
+  int *wn = alloc_into<int>(pool, nnz);      // its map (k_support_cycle)
   float *sb = alloc_into<float>(pool, nnz);
   double *c0 = alloc_into<double>(pool, nnz);
   int *done = alloc_into<int>(pool, B), *mode = alloc_into<int>(pool, B);
@@ -390,26 +410,49 @@ void sinkhorn_batched_dense(hipStream_t st, const DenseKernel &dk, const double 
       lu[k] = mode[t] ? (float)(u - 1.0) : (float)u;
       W[i + t * ldm] = f2bf(mode[t] ? (float)((double)al[k] * (u - 1.0)) : (float)((double)al[k] * u));
     });
+    // the K^T W update's map and factor for the cycle: beta b / sb
+    // (multiplicative, X = beta v = q / z) or the defect b - c0 (additive)
     launch_map(st, (long)nnz, [=] __device__(long p) {
       const int t = tgt_of[p];
-      if (!done[t]) lv[p] = mode[t] ? 0.0f : 1.0f;
+      const int n = colpos[t];
+      if (n < 0 || done[t]) {
+        wn[p] = -1;
+        return;
+      }
+      const bool add = mode[t] != 0;
+      lv[p] = add ? 0.0f : 1.0f;
+      wn[p] = (n << 1) | (add ? 1 : 0);
+      qz[p] = add ? (float)(b[p] - c0[p]) : (float)((double)be[p] * b[p] / (double)sb[p]);
     });
   };
 
   // keep: the last iteration before an absorption, whose u the absorption
   // folds into f (the other iterations need only W, not u itself)
   auto iterate = [&](bool keep) {
-    // Z = K^T W -> v, X
-    ktw_support([=] __device__(int p, int t, int j, float z) {
-      double v;
-      if (mode[t]) {
-        v = b[p] / (c0[p] + (double)be[p] * (double)z);
-        X[j + (long)t * ldn] = f2bf((float)((double)be[p] * (v - 1.0)));
-      } else {
-        v = b[p] / ((double)sb[p] * (double)z);
-        X[j + (long)t * ldn] = f2bf((float)((double)be[p] * v));
-      }
-      lv[p] = mode[t] ? (float)(v - 1.0) : (float)v;
+    // Z = K^T W -> v, X (v itself only on the kept iteration)
+    with_tiles<uint16_t>(nact, [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<uint16_t, CF>(st, Nc, nact, M, Kb, ldm, W, ldm,
+                            [] __device__(int, int, float) {}, act, nullptr, skb, false);
+      const int gx = (Nc + CF::BM - 1) / CF::BM;
+      const long ntile = (long)gx * ((nact + CF::BN - 1) / CF::BN);
+      auto op = [=] __device__(int p, bool add, float z) {
+        const long x = col_idx[p] + (long)tgt_of[p] * ldn;
+        const double q = (double)qz[p];
+        if (add) {
+          const double bp = b[p], bq = (double)be[p];
+          const double v = bp / ((bp - q) + bq * (double)z);
+          X[x] = f2bf((float)(bq * (v - 1.0)));
+          if (keep) lv[p] = (float)(v - 1.0);
+        } else {
+          const double xv = q / (double)z;
+          X[x] = f2bf((float)xv);
+          if (keep) lv[p] = (float)(xv / (double)be[p]);
+        }
+      };
+      hipLaunchKernelGGL((k_support_cycle<CF, decltype(op)>), dim3((nnz + 255) / 256),
+                         dim3(256), 0, st, skb.work, gx, ntile, nnz, col_idx, wn, op);
+      check_launch();
     });
     // Y = K X -> u, W
     if (M % 4 == 0) {

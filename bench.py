@@ -55,10 +55,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def c2_gemm(torch, dev, stream):
+def c2_gemm(torch, dev, stream, product_only=True):
     """The dominant kernel of the MFMA path at its full shape: the BF16 product
     with M = K = 4096 (the cost's points) and N = 1024 (the targets), FP32
-    accumulation, deferred epilogue (as the path runs Y = K X). Returns
+    accumulation. product_only: the product kernel alone writes its FP32 tile
+    into the workspace, as the path launches it before its own update kernels;
+    otherwise the generic split-k epilogue follows and writes D. Returns
     (launch, flops per launch)."""
     from rwig_amd._lib import check, lib
     gM, gN, gK = 4096, 1024, 4096
@@ -71,8 +73,8 @@ def c2_gemm(torch, dev, stream):
 
     def launch():
         check(lib().rwig_gemm_tn_bf16_dev(gM, gN, gK, Kt.data_ptr(), gK, Wt.data_ptr(), gK,
-                                          Dt.data_ptr(), gM, GEMM_KSPLIT, wk.data_ptr(),
-                                          stream.cuda_stream))
+                                          None if product_only else Dt.data_ptr(), gM,
+                                          GEMM_KSPLIT, wk.data_ptr(), stream.cuda_stream))
     return launch, 2.0 * gM * gN * gK
 
 
@@ -84,7 +86,7 @@ def gemm_only():
     from rwig_amd._lib import check, lib
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
-    launch, _ = c2_gemm(torch, dev, stream)
+    launch, _ = c2_gemm(torch, dev, stream, product_only=False)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
     dst = torch.empty_like(src)
     for _ in range(5):
@@ -575,9 +577,9 @@ def main():
                                        "product, copy-calibrated "
                                        "(profiles/c2_gemm_hbm_traffic.md)",
                      "alg_bytes_per_launch": (4096 * 4096 + 1024 * 4096) * 2 + 4096 * 1024 * 4,
-                     "kernel": "k_gemm_tn<bf16> + k_gemm_splitk_epilogue (4096 x 1024 x 4096, "
-                               "deferred epilogue, as the MFMA path runs Y = K X at 1024 "
-                               "targets)",
+                     "kernel": "k_gemm_tn<bf16> (4096 x 1024 x 4096, 128 x 128 tiles, "
+                               "FP32 tile into the workspace, as the MFMA path launches its "
+                               "products at 1024 targets before its update kernels)",
                      "kernel_ms": gemm_ms, "alg_flops_per_launch": gemm_flops},
         "cpu_baseline": cpu,
         **extra,

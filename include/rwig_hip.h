@@ -228,9 +228,11 @@ int rwig_sinkhorn_batched_mfma_dev(void *handle, const double *a, int B,
 /* ks >= 1 splits k over ks workgroups per tile; with a workspace (work,
  * rwig_gemm_workspace_bytes(dtype, M, N, ks) bytes, dtype 0 bf16, 1 f64)
  * the partial tiles are summed in split order by a second kernel, which also
- * writes D; without one (work null, ks 1) the product kernel writes D. The
- * tile shape follows N (smaller tiles for fewer columns); for a given ks every
- * shape gives the same result bit for bit. */
+ * writes D; without one (work null, ks 1) the product kernel writes D. With
+ * a workspace and D null only the product runs: the partial tiles stay in
+ * work, as the batched solvers run their products before their own updates.
+ * The tile shape follows N (smaller tiles for fewer columns); for a given ks
+ * every shape gives the same result bit for bit. */
 int rwig_gemm_workspace_bytes(int dtype, int M, int N, int ks, long *bytes);
 int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda,
                           const void *B, long ldb, float *D, long ldd, int ks,

@@ -18,7 +18,7 @@ int rwig_gemm_workspace_bytes(int dtype, int M, int N, int ks, long *bytes) {
 int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda, const void *B,
                           long ldb, float *D, long ldd, int ks, void *work, void *stream) {
   return guarded("rwig_gemm_tn_bf16_dev", [&] {
-    RWIG_REQUIRE(A && B && D, "null argument");
+    RWIG_REQUIRE(A && B && (D || work), "null argument");
     SplitK<uint16_t> sk;
     sk.ks = ks;
     sk.work = (float *)work;
@@ -26,7 +26,7 @@ int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda, const vo
       gemm_tn<uint16_t, decltype(cfg)>(
           (hipStream_t)stream, M, N, K, (const uint16_t *)A, lda, (const uint16_t *)B, ldb,
           [=] __device__(int m, int n, float v) { D[m + (long)n * ldd] = v; }, nullptr, nullptr,
-          sk);
+          sk, D != nullptr);
     });
   });
 }
@@ -34,7 +34,7 @@ int rwig_gemm_tn_bf16_dev(int M, int N, int K, const void *A, long lda, const vo
 int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda, const double *B,
                          long ldb, double *D, long ldd, int ks, void *work, void *stream) {
   return guarded("rwig_gemm_tn_f64_dev", [&] {
-    RWIG_REQUIRE(A && B && D, "null argument");
+    RWIG_REQUIRE(A && B && (D || work), "null argument");
     SplitK<double> sk;
     sk.ks = ks;
     sk.work = (double *)work;
@@ -42,7 +42,7 @@ int rwig_gemm_tn_f64_dev(int M, int N, int K, const double *A, long lda, const d
       gemm_tn<double, decltype(cfg)>(
           (hipStream_t)stream, M, N, K, A, lda, B, ldb,
           [=] __device__(int m, int n, double v) { D[m + (long)n * ldd] = v; }, nullptr, nullptr,
-          sk);
+          sk, D != nullptr);
     });
   });
 }

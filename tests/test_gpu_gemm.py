@@ -161,3 +161,29 @@ def test_every_tile_shape_gives_the_same_columns(dtype, K, ks, deferred):
         part = run(A, Bt[:n].contiguous(), M, n, K, ks, deferred)
         torch.cuda.synchronize()
         assert torch.equal(part, full[:n]), (dtype, K, n)
+
+
+def test_bf16_product_only_leaves_the_tiles_in_the_workspace():
+    """D null with a workspace: only the product runs, and the workspace holds
+    the FP32 tiles (column-major within each 128 x 128 tile at > 512 columns)
+    that the deferred epilogue would have written to D. Without a workspace
+    a null D is refused."""
+    M, N, K = 256, 600, 256
+    g = torch.Generator(device="cpu").manual_seed(7)
+    A = torch.randn((M, K), generator=g).to(torch.bfloat16).cuda()
+    Bt = torch.randn((N, K), generator=g).to(torch.bfloat16).cuda()
+    D = run_bf16(A, Bt, M, N, K, 1, True)
+    w = workspace(0, M, N, 1)
+    st = torch.cuda.current_stream().cuda_stream
+    check(lib().rwig_gemm_tn_bf16_dev(M, N, K, A.data_ptr(), K, Bt.data_ptr(), K, None, M, 1,
+                                      w.data_ptr(), st))
+    torch.cuda.synchronize()
+    wf = w.view(torch.float32).cpu()
+    gx, T = (M + 127) // 128, 128
+    m = torch.arange(M).view(1, M)
+    n = torch.arange(N).view(N, 1)
+    idx = ((n // T) * gx + m // T) * T * T + (n % T) * T + m % T
+    assert torch.equal(wf[idx], D.cpu())
+    rc = lib().rwig_gemm_tn_bf16_dev(M, N, K, A.data_ptr(), K, Bt.data_ptr(), K, None, M, 1,
+                                     None, st)
+    assert rc != 0

@@ -347,7 +347,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--targets", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--cpu-sample", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the barycenter-gradient, WDL and serving measurements")

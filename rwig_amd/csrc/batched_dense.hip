@@ -7,9 +7,9 @@
 // alpha = exp(f/reg), beta = exp(g/reg); the absorbed plan of target t is
 // P0_ij = alpha_i K_ij beta_j and the current plan P = diag(u) P0 diag(v).
 //
-// Per iteration (BF16 MFMA, FP32 accumulation; the Sinkhorn update reads the
-// product's FP32 tile: at the support entries only for Z, in the GEMM's
-// deferred epilogue for Y):
+// Per iteration (BF16 MFMA, FP32 accumulation; each product writes its FP32
+// tile and an update kernel reads it: k_support_cycle at the support entries
+// only for Z, k_rows4 four rows per thread for Y):
 //   Z = K^T W  ->  v on the supports, X = beta .* (v or v - 1) (zero off S_t)
 //   Y = K X    ->  u,                 W = alpha .* (u or u - 1)
 // Two forms of the update, chosen per target at every absorption:

@@ -262,16 +262,18 @@ struct Group {
                     splitk_work_elems<double>(N, c, ks_c()));
   }
 
+  // Without a k split (small n) the product kernel runs the update itself:
+  // one launch less per product, and the same value reaches the update.
   SplitK<uint16_t> skb(int ks) const {
     SplitK<uint16_t> k;
     k.ks = ks;
-    k.work = wk_b.get();
+    k.work = ks > 1 ? wk_b.get() : nullptr;
     return k;
   }
   SplitK<double> skd(int ks) const {
     SplitK<double> k;
     k.ks = ks;
-    k.work = wk_d.get();
+    k.work = ks > 1 ? wk_d.get() : nullptr;
     return k;
   }
 

@@ -166,6 +166,30 @@ def test_mfma_path_reaches_tolerance_and_matches_fp64(absorb_every):
         assert abs(out[4][t, 1] - rl) <= 1e-12 * mag, (t, out[4][t, 1], rl)
 
 
+def test_mfma_path_rows_not_a_multiple_of_4():
+    """M = 301 takes the one-row-per-thread form of the Y = K X update (the
+    4-row form needs M % 4 == 0). Every target must reach the tolerance, with
+    the error, transport cost and regularised loss of the plan recomputed on
+    the host, and the cost within 1e-5 of the FP64 batched form's."""
+    C, a, sup, wts, _ = make(43, 301, 7, 10, 60, 0.02)
+    reg, tol = 0.02, 1e-6
+    tb = RB.TargetBatch(sup, wts)
+    Cd, ad = to_dev(C, a)
+    ref = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64", err="l1",
+                                   zero_tol=tol, max_iter=5000), tb)
+    out = host(RB.sinkhorn_batched_mfma(Cd, ad, tb, reg, zero_tol=tol, max_iter=5000,
+                                        absorb_every=10), tb)
+    for t in range(tb.B):
+        _, c_ref = plan_l1(C, a, sup[t], wts[t], ref[0][t], ref[1][t], reg)
+        e, c = plan_l1(C, a, sup[t], wts[t], out[0][t], out[1][t], reg)
+        print("mfma M=301", t, "iters", out[2][t], "err", e, flush=True)
+        assert e <= tol * 1.0001, (t, e)
+        assert abs(out[3][t] - e) <= 1e-9 + 1e-6 * e
+        assert abs(c - c_ref) <= 1e-5 * abs(c_ref)
+        _, rl, mag = plan_loss(C, sup[t], out[0][t], out[1][t], reg)
+        assert abs(out[4][t, 1] - rl) <= 1e-12 * mag, (t, out[4][t, 1], rl)
+
+
 def test_mfma_path_alone_equals_in_mix():
     C, a, sup, wts, reg = make(23, 600, 6, 10, 60, 0.02)
     Cd, ad = to_dev(C, a)

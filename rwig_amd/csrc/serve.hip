@@ -407,12 +407,17 @@ struct Group {
     if (phase == 0) RWIG_HIP_CHECK(hipEventRecord(ev, st));
   }
 
+  // Column count for the tile policy: at most 32 columns take the 32 x 32
+  // tiles here, not the 16 x 32 ones. Those pay off on the direct-to-LDS path
+  // only; ragged n takes the register path, where they measured 7% slower.
+  int tile_cols() const { return std::max(cap, 33); }
+
   // A product over the running columns (act, count from the device lists), in
   // the tiles the column capacity takes.
   template <typename T, class Op>
   void product(hipStream_t st, int rows, int kdim, const T *A, long lda, const T *Bm, long ldb,
                Op op, const SplitK<T> &sk) {
-    with_tiles<T>(cap, [&](auto cfg) {
+    with_tiles<T>(tile_cols(), [&](auto cfg) {
       gemm_tn<T, decltype(cfg)>(st, rows, cap, kdim, A, lda, Bm, ldb, op, d_act(), d_cnt() + kNact,
                                 sk);
     });
@@ -529,7 +534,7 @@ struct Group {
       const long t = act[c];
       Xp[j + t * ldni] = f2bf(bep[j + t * Ni]);
     });
-    with_tiles<uint16_t>(cp, [&](auto cfg) {
+    with_tiles<uint16_t>(tile_cols(), [&](auto cfg) {
       using CF = decltype(cfg);
       gemm_tn<uint16_t, CF>(
           st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,

@@ -303,6 +303,18 @@ int rwig_softmin_workspace_bytes(int which, int M, int N, int S, int precision,
 int rwig_softmin_rows_dev(const void *C, int M, int N, long ldc, const void *F,
                           long ldf, const void *G, long ldg, int S, double reg,
                           void *out, int precision, void *work, void *stream);
+/* rwig_softmin_rows_dev without the workspace argument: it allocates
+ * nothing. No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * The chunk partials live in a static 32 MiB device array, one per device,
+ * held until the process ends; a pass whose partials
+ * (rwig_softmin_workspace_bytes with which = 0) exceed 32 MiB is refused with
+ * RWIG_ERR_INVALID. Every call on a device shares that array: calls that
+ * overlap on one device (other streams, other threads) race. */
+int rwig_softmin_rows_noalloc_dev(const void *C, int M, int N, long ldc,
+                                  const void *F, long ldf, const void *G,
+                                  long ldg, int S, double reg, void *out,
+                                  int precision, void *stream);
 int rwig_softmin_cols_dev(const void *CT, int M, int N, long ldct,
                           const void *F, long ldf, const void *G, long ldg,
                           int S, double reg, void *out, int precision,

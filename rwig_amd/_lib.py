@@ -65,6 +65,8 @@ SIGNATURES = {
     "rwig_softmin_workspace_bytes": ([_i, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_long)], _i),
     "rwig_softmin_rows_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _vp,
                                ctypes.c_long, _i, _d, _vp, _i, _vp, _vp], _i),
+    "rwig_softmin_rows_noalloc_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _vp,
+                                       ctypes.c_long, _i, _d, _vp, _i, _vp], _i),
     "rwig_softmin_cols_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _vp,
                                ctypes.c_long, _i, _d, _vp, _i, _vp, _vp], _i),
     "rwig_serve_create": ([_d, _i, _i, _d, _i, ctypes.POINTER(ctypes.c_void_p)], _i),

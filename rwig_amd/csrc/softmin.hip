@@ -2,12 +2,42 @@
 // log-domain solvers, kernels.hpp) on device pointers: for callers that build
 // their own log-domain iterations, and for measuring the pass alone. The
 // calls enqueue on the stream and return; the caller provides the workspace
-// of the chunk partials.
+// of the chunk partials, except in rwig_softmin_rows_noalloc_dev, whose
+// partials live in the file-scope device array below.
+#include <mutex>
+#include <unordered_map>
+
 #include "kernels.hpp"
 
 using namespace rwig;
 
+// Chunk partials (running minimum and scaled sum per chunk, row and slice) of
+// rwig_softmin_rows_noalloc_dev: 32 MiB per device, enough for FP64 at
+// M = N = 4096 with S = 16 (2 x 32 chunks x 16 x 4096 x 8 B). Every call on a
+// device shares its array; calls that overlap on one device race. Nothing in
+// it needs to start at zero: the pass writes every partial it reads.
+static constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current. The array's name is a host-side stand-in
+// in host code: kernels must be given this address instead.
+void *scratch_address() {
+  static std::mutex mu;
+  static std::unordered_map<int, void *> addr;
+  int dev = 0;
+  RWIG_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(dev);
+  if (it != addr.end()) return it->second;
+  void *p = nullptr;
+  RWIG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  RWIG_REQUIRE(p != nullptr, "no device address for the soft-min scratch array");
+  addr.emplace(dev, p);
+  return p;
+}
 
 template <typename T> size_t part_elems(int which, int M, int N, int S) {
   return which == 0 ? lse_row_part_size(M, N, S) : col_t_part_size(M, N, S);
@@ -61,6 +91,28 @@ int rwig_softmin_rows_dev(const void *C, int M, int N, long ldc, const void *F, 
     check_args(M, N, S, reg, precision);
     RWIG_REQUIRE(C && F && G && out && work && ldc >= M && ldf >= M && ldg >= N,
                  "invalid argument");
+    if (precision == 0)
+      rows<double>((hipStream_t)stream, (const double *)C, M, N, ldc, (const double *)F, ldf,
+                   (const double *)G, ldg, S, reg, (double *)out, (double *)work);
+    else
+      rows<float>((hipStream_t)stream, (const float *)C, M, N, ldc, (const float *)F, ldf,
+                  (const float *)G, ldg, S, reg, (float *)out, (float *)work);
+  });
+}
+
+int rwig_softmin_rows_noalloc_dev(const void *C, int M, int N, long ldc, const void *F,
+                                  long ldf, const void *G, long ldg, int S, double reg,
+                                  void *out, int precision, void *stream) {
+  return guarded("rwig_softmin_rows_noalloc_dev", [&] {
+    check_args(M, N, S, reg, precision);
+    RWIG_REQUIRE(C && F && G && out && ldc >= M && ldf >= M && ldg >= N, "invalid argument");
+    const size_t need = 2 * (precision == 0 ? part_elems<double>(0, M, N, S) * sizeof(double)
+                                            : part_elems<float>(0, M, N, S) * sizeof(float));
+    RWIG_REQUIRE(need <= kScratchBytes,
+                 "the pass needs " + std::to_string(need) + " bytes of chunk partials, more than "
+                 "the " + std::to_string(kScratchBytes) + "-byte static scratch array holds; "
+                 "use rwig_softmin_rows_dev with a workspace");
+    void *work = scratch_address();
     if (precision == 0)
       rows<double>((hipStream_t)stream, (const double *)C, M, N, ldc, (const double *)F, ldf,
                    (const double *)G, ldg, S, reg, (double *)out, (double *)work);

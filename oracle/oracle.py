@@ -224,6 +224,17 @@ def wdl(Y, C, reg, S, Alpha0, Lambda0, batch_size, epochs, sinkhorn_mode=1,
     return {"A": A, "W": W, "Yhat": Yhat}
 
 
+def wdl_init_mt19937(seed, N, S, M):
+    """Initial (Alpha0 N x S, Lambda0 S x M) of the reference's CUDA WDL path
+    (src/cuda_wdl.cu:592-593, :677-689): std::mt19937(seed) standard normals,
+    Alpha first, column-major."""
+    Alpha0 = np.zeros((N, S), order="F")
+    Lambda0 = np.zeros((S, M), order="F")
+    lib().oracle_wdl_init_mt19937(int(seed), int(N), int(S), int(M), _ptr(Alpha0),
+                                  _ptr(Lambda0))
+    return Alpha0, Lambda0
+
+
 def spectral_norm(X):
     X = _f64(X)
     if X.ndim == 1:

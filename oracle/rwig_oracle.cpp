@@ -18,6 +18,8 @@
 //   src/wdl_impl.cpp                (_train_batch_batched, _compute_serial)
 //   inst/include/wdl_impl.hpp       (_softmax, _optimize)
 //   inst/include/optimizer.hpp      (sgd, adam, adamw)
+//   src/cuda_wdl.cu:592-593, :677-689 (the CUDA path's seeded initial Alpha
+//                                    and Lambda)
 //   src/sinkhorn.cpp, src/barycenter.cpp (output assembly, return codes)
 //
 // Armadillo semantics reproduced here:
@@ -36,6 +38,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -1237,6 +1240,18 @@ int oracle_wdl(const double *Y, const double *C, int N, int M, int S,
     }
   }
   return 0;
+}
+
+// Initial Alpha (N x S) and Lambda (S x M) of the reference's CUDA WDL path
+// (src/cuda_wdl.cu:592-593, :677-689): std::mt19937(seed) and
+// std::normal_distribution<double>(0, 1), N*S draws into Alpha in column-major
+// order first, then S*M draws into Lambda.
+void oracle_wdl_init_mt19937(int seed, int N, int S, int M, double *Alpha0,
+                             double *Lambda0) {
+  std::mt19937 rng(seed);
+  std::normal_distribution<double> randn(0.0, 1.0);
+  for (long k = 0; k < (long)N * S; ++k) Alpha0[k] = randn(rng);
+  for (long k = 0; k < (long)S * M; ++k) Lambda0[k] = randn(rng);
 }
 
 // Largest singular value of an m x s column-major matrix (Armadillo

@@ -1,7 +1,8 @@
 """GPU tests of Wasserstein Dictionary Learning (rwig_wdl*, librwig_hip.so)
 against the FP64 oracle (src/wdl_impl.cpp restated).
 
-FP64: batched gradient and the whole fit (A, W, Yhat) to 1e-9 relative.
+FP64: batched gradient and the whole fit (A, W, Yhat) to 1e-9 relative, from
+explicit initial parameters and from the seeded mt19937 draw.
 FP32: batched gradient to 1e-4 relative (FP32 Gibbs products, stated
 tolerance)."""
 import numpy as np
@@ -81,6 +82,21 @@ def test_wdl_fit_other_optimizers(opt):
     kw = dict(batch_size=4, epochs=1, max_iter=8, optimizer=opt, eta=0.05)
     r = rwig_amd.wdl(Y, C, 0.5, S, init=(Alpha0, Lambda0), **kw)
     o = O.wdl(Y, C, 0.5, S, Alpha0, Lambda0, **kw)
+    for k in ["A", "W", "Yhat"]:
+        assert rel(r[k], o[k]) < 1e-9, k
+
+
+@pytest.mark.parametrize("seed,opt", [(7, 2), (123, 1)])
+def test_wdl_seeded_entry_point_matches_oracle(seed, opt):
+    # rwig_wdl draws Alpha0, Lambda0 as the reference's CUDA path does
+    # (src/cuda_wdl.cu:592-593, :677-689); the oracle's mt19937 helper is
+    # pinned to the polar method on MT19937 in the CPU tests
+    N, M, S = 30, 7, 3
+    C, Y, _ = wdl_problem(5 + seed, N, M, S)
+    kw = dict(batch_size=3, epochs=2, max_iter=6, optimizer=opt, eta=0.02)
+    r = rwig_amd.wdl(Y, C, 0.4, S, seed=seed, **kw)
+    Alpha0, Lambda0 = O.wdl_init_mt19937(seed, N, S, M)
+    o = O.wdl(Y, C, 0.4, S, Alpha0, Lambda0, **kw)
     for k in ["A", "W", "Yhat"]:
         assert rel(r[k], o[k]) < 1e-9, k
 

@@ -176,8 +176,10 @@ def test_threaded_log_equals_serial():
 
 
 def _fd(fun, x, h=1e-6):
-    g = np.zeros_like(x)
-    flat = x.reshape(-1)
+    # C-ordered work arrays: reshape(-1) of a Fortran-ordered array is a copy,
+    # and writes through it would be lost
+    g = np.zeros(x.shape)
+    flat = np.ascontiguousarray(x).reshape(-1)
     for k in range(flat.size):
         xp = flat.copy(); xp[k] += h
         xm = flat.copy(); xm[k] -= h
@@ -321,6 +323,155 @@ def test_wdl_end_to_end_runs_and_is_normalised():
     np.testing.assert_allclose(out["A"].sum(0), 1, rtol=1e-13)
     np.testing.assert_allclose(out["W"].sum(0), 1, rtol=1e-13)
     np.testing.assert_allclose(out["Yhat"].sum(0), 1, rtol=1e-12)
+
+
+# ------------------------------------------------- the WDL loop, pinned piece by piece
+#
+# The loop of _compute_serial (src/wdl_impl.cpp:254-347) is checked against
+# pieces pinned elsewhere: the batch loss comes from the golden-pinned
+# parallel barycenter (unnormalised b_L = V_L .* K^T U_L after exactly L
+# iterations at zero_tol 0), its gradient from central differences, and each
+# step from the closed-form-pinned optimizer. A wrong batch slice, a wrong
+# batch size for the last batch, a g_Lambda broadcast over the batch's columns
+# only, or a mean over B instead of D fails these.
+
+
+def _wdl_problem(seed, N, M):
+    rng = np.random.default_rng(seed)
+    x = rng.random((N, 2))
+    C = ((x[:, None] - x[None]) ** 2).sum(-1)
+    Y = np.stack([_rand_simplex(rng, N) for _ in range(M)], 1)
+    return C, Y, rng
+
+
+def _wdl_batch_loss(C, reg, L, Yb, Alpha, Lam_b):
+    """mean_d ||b_L,d - y_d||^2 with b_L,d the unnormalised barycenter after
+    L parallel-barycenter iterations (the loss whose gradient
+    src/wdl_impl.cpp:87-247 returns)."""
+    A = O.softmax_cols(Alpha)
+    W = O.softmax_cols(Lam_b)
+    K = np.exp(-C / reg)
+    tot = 0.0
+    for d in range(Yb.shape[1]):
+        r = O.barycenter_parallel(A, C, W[:, d], reg=reg, maxiter=L, zerotol=0.0)
+        assert r["iter"] == L
+        bL = r["V"][:, 0] * (K.T @ r["U"][:, 0])
+        tot += np.sum((bL - Yb[:, d]) ** 2)
+    return tot / Yb.shape[1]
+
+
+def _wdl_fd_grads(C, reg, L, Yb, Alpha, Lam_b, h=1e-6):
+    """Central differences: d/dAlpha, and d/dlambda along a shift of row s of
+    every batch column of Lambda at once (the reference's g_lambda)."""
+    f = lambda X: _wdl_batch_loss(C, reg, L, Yb, X, Lam_b)  # noqa: E731
+    gA = _fd(f, Alpha, h)
+    S = Lam_b.shape[0]
+    gl = np.zeros(S)
+    for s in range(S):
+        e = np.zeros_like(Lam_b)
+        e[s] = h
+        gl[s] = (_wdl_batch_loss(C, reg, L, Yb, Alpha, Lam_b + e)
+                 - _wdl_batch_loss(C, reg, L, Yb, Alpha, Lam_b - e)) / (2 * h)
+    return gA, gl
+
+
+def _wdl_by_pieces(C, Y, reg, S, Alpha0, Lambda0, B, E, L, opt, eta):
+    """The epoch x batch loop composed from the pinned pieces."""
+    N, M = Y.shape
+    Alpha, Lam = Alpha0.copy(), Lambda0.copy()
+    mA, vA, mL, vL = (np.zeros(N * S), np.zeros(N * S), np.zeros(S * M), np.zeros(S * M))
+    sA = sL = 0
+    batches = -(-M // B)
+    sizes = []
+    for _ in range(E):
+        for bid in range(batches):
+            lo, hi = bid * B, min(M, bid * B + B)
+            sizes.append(hi - lo)
+            gA, gl = _wdl_fd_grads(C, reg, L, Y[:, lo:hi], Alpha, Lam[:, lo:hi])
+            a = Alpha.ravel(order="F").copy()
+            sA = O.optimizer_step(opt, a, gA.ravel(order="F"), mA, vA, sA, eta=eta)
+            Alpha = a.reshape((N, S), order="F")
+            lam = Lam.ravel(order="F").copy()
+            gL = np.repeat(gl[:, None], M, axis=1)  # g_Lambda = g_lambda 1_M^T
+            sL = O.optimizer_step(opt, lam, gL.ravel(order="F"), mL, vL, sL, eta=eta)
+            Lam = lam.reshape((S, M), order="F")
+    return Alpha, Lam, sizes
+
+
+@pytest.mark.parametrize("opt,eta", [(0, 0.5), (1, 0.05)])
+def test_wdl_one_batch_one_epoch_is_one_step_on_the_fd_gradient(opt, eta):
+    N, M, S, L, reg = 7, 4, 3, 5, 0.4
+    C, Y, rng = _wdl_problem(21, N, M)
+    Alpha0 = rng.standard_normal((N, S))
+    Lambda0 = rng.standard_normal((S, M))
+    gA, gl = _wdl_fd_grads(C, reg, L, Y, Alpha0, Lambda0)
+    assert np.min(np.abs(gA)) > 1e-6 and np.min(np.abs(gl)) > 1e-6  # Adam's step is resolved
+    step = (lambda g: g) if opt == 0 else (lambda g: g / (np.abs(g) + 1e-8))
+    A1 = O.softmax_cols(Alpha0 - eta * step(gA))
+    W1 = O.softmax_cols(Lambda0 - eta * step(gl)[:, None])
+    out = O.wdl(Y, C, reg, S, Alpha0, Lambda0, batch_size=M, epochs=1, max_iter=L,
+                zero_tol=1e-9, optimizer=opt, eta=eta)
+    np.testing.assert_allclose(out["A"], A1, rtol=1e-7, atol=1e-12)
+    np.testing.assert_allclose(out["W"], W1, rtol=1e-7, atol=1e-12)
+    # inference: each document's normalised barycenter of the trained atoms
+    for m in range(M):
+        r = O.barycenter_parallel(A1, C, W1[:, m], reg=reg, maxiter=L, zerotol=1e-9)
+        np.testing.assert_allclose(out["Yhat"][:, m], r["b"], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("opt", [0, 2])
+def test_wdl_ragged_last_batch_and_lambda_broadcast_over_two_epochs(opt):
+    # M % B != 0: batches of 3 and 2 documents; every column of Lambda moves
+    # by the same g_lambda after every batch, whether or not it is in it
+    N, M, S, L, reg, B, E = 6, 5, 2, 4, 0.3, 3, 2
+    C, Y, rng = _wdl_problem(22, N, M)
+    Alpha0 = rng.standard_normal((N, S))
+    Lambda0 = rng.standard_normal((S, M))
+    eta = 0.5 if opt == 0 else 0.05
+    Alpha, Lam, sizes = _wdl_by_pieces(C, Y, reg, S, Alpha0, Lambda0, B, E, L, opt, eta)
+    assert sizes == [3, 2, 3, 2]
+    out = O.wdl(Y, C, reg, S, Alpha0, Lambda0, batch_size=B, epochs=E, max_iter=L,
+                optimizer=opt, eta=eta)
+    np.testing.assert_allclose(out["A"], O.softmax_cols(Alpha), rtol=1e-7, atol=1e-12)
+    np.testing.assert_allclose(out["W"], O.softmax_cols(Lam), rtol=1e-7, atol=1e-12)
+    # the broadcast: Lambda moved by one vector for all columns (SGD only:
+    # AdamW's decay scales each column by the same factor, which keeps the
+    # per-column differences, so check the differences instead)
+    d = Lam - Lambda0 if opt == 0 else Lam - (1 - eta * 0.01) ** len(sizes) * Lambda0
+    np.testing.assert_allclose(d, np.repeat(d[:, :1], M, axis=1), rtol=1e-12, atol=1e-14)
+
+
+def test_wdl_init_mt19937_is_the_polar_method_on_mt19937():
+    # numpy's legacy RandomState(seed) is MT19937 seeded by init_genrand, as
+    # std::mt19937(seed); the C++ standard fixes its 10000th output at seed 5489
+    raw = np.random.RandomState(5489)._bit_generator.random_raw(10000)
+    assert int(raw[-1]) == 4123659995
+    seed, N, S, M = 42, 9, 3, 5
+    bits = iter(np.random.RandomState(seed)._bit_generator.random_raw(4 * (N * S + S * M) + 64))
+
+    def canonical():  # 53-bit uniform from two 32-bit draws, low word first
+        lo, hi = next(bits), next(bits)
+        return (float(lo) + float(hi) * 2.0 ** 32) / 2.0 ** 64
+
+    saved = []
+
+    def normal():  # Marsaglia's polar method, second value of a pair kept
+        if saved:
+            return saved.pop()
+        while True:
+            x = 2.0 * canonical() - 1.0
+            y = 2.0 * canonical() - 1.0
+            r2 = x * x + y * y
+            if 0.0 < r2 <= 1.0:
+                break
+        mult = np.sqrt(-2.0 * np.log(r2) / r2)
+        saved.append(x * mult)
+        return y * mult
+
+    draws = np.array([normal() for _ in range(N * S + S * M)])
+    Alpha0, Lambda0 = O.wdl_init_mt19937(seed, N, S, M)
+    np.testing.assert_allclose(Alpha0.ravel(order="F"), draws[:N * S], rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(Lambda0.ravel(order="F"), draws[N * S:], rtol=1e-14, atol=1e-15)
 
 
 # ------------------------------------------------------------- cost / histograms

@@ -205,8 +205,9 @@ int rwig_gibbs_lowp_dev(const double *C, long ldc, int M, int N, double reg,
  * and exp(-C/reg) .* C in FP64 for the transport cost (rwig_dense_kernel_create,
  * one per cost matrix and reg); the solve reads the cost only through it.
  * tgt_of[p] is the target owning support entry p. Other arguments as
- * rwig_sinkhorn_batched_dev. No counterpart in the reference (it solves one
- * problem per call). */
+ * rwig_sinkhorn_batched_dev. A target whose error turns non-finite stops
+ * there and reports that error (the reference's return_status 2). No
+ * counterpart in the reference (it solves one problem per call). */
 int rwig_dense_kernel_create(const double *C, long ldc, int M, int Nc,
                              double reg, void *stream, void **handle);
 int rwig_dense_kernel_destroy(void *handle);
@@ -337,8 +338,15 @@ int rwig_softmin_cols_dev(const void *CT, int M, int N, long ldct,
  * budget (cost matrices, cached Gibbs kernels, group columns) holds them.
  * Stopping: L1 marginal error of the absorbed plans <= zero_tol, tested at
  * each absorption, or max_iter reached (iteration counts are multiples of
- * absorb_every). A request's result does not depend on the other requests
- * in flight. Entries of a, b and A must be positive. Calls are thread-safe. */
+ * absorb_every, or max_iter). A request's result does not depend on the
+ * other requests in flight. Entries of a, b and A must be finite and
+ * non-negative, each measure with positive mass; zero entries are off the
+ * support, as the reference's support reduction
+ * (src/sinkhorn_impl_vanilla.cpp:29-35): their potentials are -inf and their
+ * plan entries zero. Submission refuses (RWIG_ERR_INVALID) a reg so small
+ * that a used row or column of the cost has min C / reg > 60, where the
+ * shared low-precision Gibbs kernel would underflow; the log-domain entry
+ * points solve those. Calls are thread-safe. */
 #define RWIG_SERVE_EAGER 1 /* flags: launch every kernel of a cycle directly
                              instead of replaying recorded HIP graphs */
 int rwig_serve_create(double hbm_budget_bytes, int max_group_columns,
@@ -362,15 +370,21 @@ int rwig_serve_submit_barycenter(void *session, int cost_id, double reg,
                                  int max_iter, double zero_tol,
                                  long *request_id);
 /* Waits until the request has finished (timeout_s < 0: no limit); state:
- * 0 queued, 1 running, 2 done, 3 failed. */
+ * 0 queued, 1 running, 2 done, 3 failed (engine failure, or a marginal
+ * error that turned non-finite). */
 int rwig_serve_wait(void *session, long request_id, double timeout_s,
                     int *state);
 /* Results of a finished request, which is then forgotten. Sinkhorn: x = f
  * (M), y = g (N) potentials, loss = sum C.*P + reg sum P.*(log P - 1).
  * Barycenter: x = b (N, normalised), y unused. latency_ms: submit to
- * completion. */
+ * completion. status (may be null): the reference's return_status
+ * (src/sinkhorn.cpp:88-95): 0 err <= zero_tol, 1 max_iter reached, 2 the
+ * solve broke down (non-finite error; the call still returns RWIG_OK and the
+ * outputs as they stood). A request failed by the engine itself (HIP error,
+ * session destroyed) returns RWIG_ERR_HIP. */
 int rwig_serve_result(void *session, long request_id, double *x, double *y,
-                      double *loss, int *iter, double *err, double *latency_ms);
+                      double *loss, int *iter, double *err, double *latency_ms,
+                      int *status);
 /* out[0..14]: completed, failed, queued, running, p50 latency (ms),
  * p99 latency (ms), mean latency (ms), solves/s (first submit to last
  * completion), HBM bytes in use, peak HBM bytes, budget, engine cycles,

@@ -78,7 +78,7 @@ SIGNATURES = {
     "rwig_serve_submit_barycenter": ([_vp, _i, _d, _dp, _i, _i, _dp, _i, _d,
                                       ctypes.POINTER(ctypes.c_long)], _i),
     "rwig_serve_wait": ([_vp, ctypes.c_long, _d, _ip], _i),
-    "rwig_serve_result": ([_vp, ctypes.c_long, _dp, _dp, _dp, _ip, _dp, _dp], _i),
+    "rwig_serve_result": ([_vp, ctypes.c_long, _dp, _dp, _dp, _ip, _dp, _dp, _ip], _i),
     "rwig_serve_stats": ([_vp, _dp, _i], _i),
     "rwig_serve_reset_stats": ([_vp], _i),
 }

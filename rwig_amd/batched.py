@@ -102,6 +102,18 @@ def sinkhorn_batched(C, a, targets: TargetBatch, reg, precision="fp64", max_iter
     return out
 
 
+def return_status(out, max_iter, zero_tol):
+    """Per-target return_status of a finished batched solve, as the reference's
+    shim computes it (src/sinkhorn.cpp:88-95): 0 where err <= zero_tol, 1 where
+    the target ran max_iter iterations, else 2 (a target whose error turned
+    non-finite stops and lands here). Call after the solve's stream has
+    finished."""
+    err, it = out["err"], out["iter"]
+    st = torch.full_like(it, 2)
+    st = torch.where(it >= int(max_iter), torch.ones_like(st), st)
+    return torch.where(err <= float(zero_tol), torch.zeros_like(st), st)
+
+
 class DenseKernel:
     """exp(-C/reg) in BF16 and FP64, both orientations, and exp(-C/reg) .* C in
     FP64, for the MFMA path (rwig_dense_kernel_create). Built once per cost

@@ -21,6 +21,14 @@
 // starts its absorption cycle at admission), so its result is the same
 // alone and in any mix.
 //
+// Zero entries of a, b (Sinkhorn) or A (barycenter) are handled as the
+// reference's support reduction (src/sinkhorn_impl_vanilla.cpp:29-35):
+// those rows and columns keep a zero scaling, their potentials are -inf and
+// their plan entries zero, so they drop out of every product. A request
+// whose Gibbs kernel would underflow (a row or column of the cost with
+// min C / reg above kMaxCostOverReg) is refused at submission, and one whose
+// marginal error turns non-finite ends as failed with status 2.
+//
 // Algorithms: log-domain Sinkhorn (src/sinkhorn_impl_log.cpp:18-118) and the
 // parallel barycenter (src/barycenter_impl_parallel.cpp:60-151), written in
 // the absorbed form: potentials f, g in FP64 and scalings u, v relative to
@@ -48,6 +56,11 @@ using Clock = std::chrono::steady_clock;
 constexpr int kSinkhorn = 0, kBarycenter = 1;
 enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 constexpr int kColPage = 32;  // column capacity granularity of a group
+// Largest min_j C_ij / reg (over a row, and min_i over a column) a request may
+// have: exp(-60) = 8.8e-27 is far inside the BF16/FP32 range of the shared
+// Gibbs kernel (smallest normal 1.2e-38 = exp(-87.3)), with room for the
+// scalings exp(f / reg) that absorb the shift.
+constexpr double kMaxCostOverReg = 60.0;
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -73,6 +86,8 @@ void launch_cols(hipStream_t st, long rows, int bound, const int *cnt, int which
 
 // indices into the device list header
 constexpr int kNact = 0, kNreq = 1, kNbar = 2;
+// per-list length after the 4-int header: act, areq, riter, abar, lim
+constexpr int kLists = 5;
 
 // Stopping test of each running request at an absorption, one block per
 // request: L1 marginal error of the absorbed plans, sum over the request's
@@ -148,6 +163,7 @@ k_serve_err(int M, int N, double reg, const int *__restrict__ areq,
 struct Cost {
   int M = 0, N = 0;
   DBuf<double> C;
+  std::vector<double> rowmin, colmin;  // min_j C_ij and min_i C_ij (host)
 };
 
 struct Gibbs {
@@ -171,6 +187,8 @@ struct Request {
   double loss = 0.0, err = 0.0;
   int iter = 0;
   int state = kQueued;
+  int status = 2;           // 0 err <= tol, 1 max_iter reached, 2 otherwise
+  bool numeric = false;     // failed by a non-finite error (results kept)
   std::string error;
   Clock::time_point t_submit, t_admit, t_done;
   Group *group = nullptr;
@@ -206,7 +224,8 @@ struct Group {
   Slab bcol, G, c0, lv, be, sb, Zq, X, Bd;   // target side (N rows)
   Slab kind, base, ns, done, mode, maxit;    // ints per column
   Slab wcol, tol, errs;                      // doubles per column
-  // device lists: header (nact, nreq, nbar, -) | act | areq | riter | abar
+  // device lists: header (nact, nreq, nbar, -) | act | areq | riter | abar |
+  // lim (per column: iterations to run in this cycle's iteration phase)
   DBuf<int> lists;
   // stopping test spread over err_blocks() blocks per request: partial sums
   // and per-request arrival counters (zero between launches)
@@ -314,7 +333,7 @@ struct Group {
       stream->sync();
       s->buf = std::move(nb);
     }
-    lists.reset(4 + 4 * (size_t)ncap);
+    lists.reset(4 + kLists * (size_t)ncap);
     errpart.reset((size_t)ncap * err_blocks());
     arrive.reset(ncap);
     int *ar = arrive.get();
@@ -323,7 +342,7 @@ struct Group {
     wk_d.reset(wk_d_elems(ncap));
     for (int *&h : hl) {
       if (h) RWIG_HIP_CHECK(hipHostFree(h));
-      RWIG_HIP_CHECK(hipHostMalloc((void **)&h, sizeof(int) * (4 + 4 * (size_t)ncap)));
+      RWIG_HIP_CHECK(hipHostMalloc((void **)&h, sizeof(int) * (4 + kLists * (size_t)ncap)));
     }
     if (hdone_pin) RWIG_HIP_CHECK(hipHostFree(hdone_pin));
     if (herr_pin) RWIG_HIP_CHECK(hipHostFree(herr_pin));
@@ -348,14 +367,20 @@ struct Group {
   // synchronisation that follows its previous upload.
   void fill_lists(int phase) {
     int *h = hl[phase];
-    int *hact = h + 4, *hreq = h + 4 + cap, *hit = h + 4 + 2 * cap, *hbar = h + 4 + 3 * cap;
+    int *hact = h + 4, *hreq = h + 4 + cap, *hit = h + 4 + 2 * cap, *hbar = h + 4 + 3 * cap,
+        *hlim = h + 4 + 4 * cap;
     nact = nreq = nbar = 0;
     for (Request *r : running) {
       hreq[nreq] = r->col;
       hit[nreq] = r->iter;
       ++nreq;
       if (r->kind == kBarycenter) hbar[nbar++] = r->col;
-      for (int s = 0; s < r->S; ++s) hact[nact++] = r->col + s;
+      // the last cycle of a request stops at max_iter exactly
+      const int lim = std::min(absorb_every, r->max_iter - r->iter);
+      for (int s = 0; s < r->S; ++s) {
+        hact[nact++] = r->col + s;
+        hlim[r->col + s] = lim;
+      }
     }
     h[kNact] = nact;
     h[kNreq] = nreq;
@@ -363,13 +388,14 @@ struct Group {
     h[3] = 0;
   }
   void upload_lists(int phase) {
-    h2d(lists.get(), hl[phase], 4 + 4 * (size_t)cap, stream->get());
+    h2d(lists.get(), hl[phase], 4 + kLists * (size_t)cap, stream->get());
   }
   const int *d_cnt() const { return lists.get(); }
   const int *d_act() const { return lists.get() + 4; }
   const int *d_areq() const { return lists.get() + 4 + cap; }
   const int *d_riter() const { return lists.get() + 4 + 2 * cap; }
   const int *d_abar() const { return lists.get() + 4 + 3 * cap; }
+  const int *d_lim() const { return lists.get() + 4 + 4 * cap; }
 
   // Runs one phase of a cycle on the stream: the list upload, then the
   // absorption (phase 0) or absorb_every iterations (phase 1). With graphs,
@@ -382,7 +408,7 @@ struct Group {
       if (phase == 0)
         absorb();
       else
-        for (int s = 0; s < absorb_every; ++s) iterate();
+        for (int s = 0; s < absorb_every; ++s) iterate(s);
     };
     if (!graphs) {
       body();
@@ -491,9 +517,18 @@ struct Group {
           *rap = ra.p<float>(), *sbp = sb.p<float>(), *Ycp = Yc.p<float>();
     uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
     int *dn = done.p<int>(), *md = mode.p<int>();
+    const int *kd = kind.p<int>();
     const DenseKernel &dk = gk->dk;
+    // rows with a_i = 0 (columns with b_j = 0) are off the support: potential
+    // -inf, scaling 0, so they drop out of every product
     launch_cols(st, Mi, cp, cnt, kNact, [=] __device__(int c, long i) {
       const long t = act[c], q = i + t * Mi;
+      if (ap[q] == 0.0) {
+        Fp[q] = -INFINITY;
+        alp[q] = 0.0f;
+        Adp[i + t * ldmi] = 0.0;
+        return;
+      }
       const double x = (double)lup[q];
       const double fk = Fp[q] + rg * (md[t] ? log1p(x) : log(x));
       Fp[q] = fk;
@@ -503,6 +538,12 @@ struct Group {
     });
     launch_cols(st, Ni, cp, cnt, kNact, [=] __device__(int c, long j) {
       const long t = act[c], q = j + t * Ni;
+      if (kd[t] == kSinkhorn && bp[q] == 0.0) {
+        Gp[q] = -INFINITY;
+        bep[q] = 0.0f;
+        Bdp[j + t * ldni] = 0.0;
+        return;
+      }
       const double x = (double)lvp[q];
       const double gq = Gp[q] + rg * (md[t] ? log1p(x) : log(x));
       Gp[q] = gq;
@@ -555,7 +596,7 @@ struct Group {
       const long t = act[c], q = i + t * Mi;
       if (dn[t]) return;
       rap[q] = (float)(r0p[q] / (double)Ycp[q]);
-      const double u = ap[q] / r0p[q];
+      const double u = ap[q] > 0.0 ? ap[q] / r0p[q] : 0.0;
       const double a_ = (double)alp[q];
       lup[q] = md[t] ? (float)(u - 1.0) : (float)u;
       Wp[i + t * ldmi] = f2bf(md[t] ? (float)(a_ * (u - 1.0)) : (float)(a_ * u));
@@ -570,12 +611,14 @@ struct Group {
 
   // One iteration over the running columns: Z = K^T W -> v, X; for the
   // barycenters the common target and v; Y = K X -> u, W.
-  void iterate() {
+  // Columns whose request has run its max_iter iterations (lim <= s) are
+  // left as they are. Rows and columns off the support keep zero scalings.
+  void iterate(int s_it) {
     hipStream_t st = stream->get();
     const int Mi = M, Ni = N, cp = cap;
     const long ldmi = ldm, ldni = ldn;
     const double rg = reg;
-    const int *abar = d_abar(), *cnt = d_cnt();
+    const int *abar = d_abar(), *cnt = d_cnt(), *lim = d_lim();
     const int *kd = kind.p<int>(), *md = mode.p<int>(), *np = ns.p<int>();
     const double *c0p = c0.p<double>(), *r0p = r0.p<double>(), *ap = acol.p<double>(),
                  *bp = bcol.p<double>(), *Gp = G.p<double>(), *wp = wcol.p<double>();
@@ -587,19 +630,21 @@ struct Group {
     product<uint16_t>(
         st, Ni, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
         [=] __device__(int j, int t, float z) {
+          if (s_it >= lim[t]) return;
           const long q = j + (long)t * Ni;
           if (kd[t] == kBarycenter) {
             Zqp[q] = z;
             return;
           }
           const double b_ = (double)bep[q];
+          const bool on = bp[q] > 0.0;
           double v;
           if (md[t]) {
-            v = bp[q] / (c0p[q] + b_ * (double)z);
+            v = on ? bp[q] / (c0p[q] + b_ * (double)z) : 0.0;
             Xp[j + (long)t * ldni] = f2bf((float)(b_ * (v - 1.0)));
             lvp[q] = (float)(v - 1.0);
           } else {
-            v = bp[q] / ((double)sbp[q] * (double)z);
+            v = on ? bp[q] / ((double)sbp[q] * (double)z) : 0.0;
             Xp[j + (long)t * ldni] = f2bf((float)(b_ * v));
             lvp[q] = (float)v;
           }
@@ -607,6 +652,7 @@ struct Group {
         skb(ks_z()));
     launch_cols(st, Ni, cp, cnt, kNbar, [=] __device__(int c, long j) {
       const long r = abar[c];
+      if (s_it >= lim[r]) return;
       const int S = np[r], m1 = md[r];
       double lb = 0.0;
       for (int s = 0; s < S; ++s) {
@@ -629,15 +675,17 @@ struct Group {
     product<uint16_t>(
         st, Mi, Ni, dk.KbT.get(), ldni, Xp, ldni,
         [=] __device__(int i, int t, float y) {
+          if (s_it >= lim[t]) return;
           const long q = i + (long)t * Mi;
           const double a_ = (double)alp[q];
+          const bool on = ap[q] > 0.0;
           double u;
           if (md[t]) {
-            u = ap[q] / (r0p[q] + a_ * (double)y);
+            u = on ? ap[q] / (r0p[q] + a_ * (double)y) : 0.0;
             Wp[i + (long)t * ldmi] = f2bf((float)(a_ * (u - 1.0)));
             lup[q] = (float)(u - 1.0);
           } else {
-            u = ap[q] / ((double)rap[q] * (double)y);
+            u = on ? ap[q] / ((double)rap[q] * (double)y) : 0.0;
             Wp[i + (long)t * ldmi] = f2bf((float)(a_ * u));
             lup[q] = (float)u;
           }
@@ -651,6 +699,13 @@ struct Group {
     hipStream_t st = stream->get();
     const long c = r.col;
     r.err = herr_pin[c];
+    // return_status as src/sinkhorn.cpp:88-95 computes it
+    if (!std::isfinite(r.err)) {
+      r.status = 2;
+      r.numeric = true;
+    } else {
+      r.status = r.err <= r.tol ? 0 : (r.iter >= r.max_iter ? 1 : 2);
+    }
     if (r.kind == kSinkhorn) {
       r.out1.resize(M);
       r.out2.resize(N);
@@ -674,12 +729,16 @@ struct Group {
       // P = exp((f 1^T + 1 g^T - C) / reg): there C + reg (log P - 1) =
       // f_i + g_j - reg, so the loss is f.r + g.c - reg sum(r) with r, c the
       // plan's exact marginals from the last absorption
+      // (entries off the support have f or g = -inf and a zero marginal:
+      // they add nothing)
       double fr = 0.0, gc = 0.0, tr = 0.0;
       for (int i = 0; i < M; ++i) {
+        if (r.r0[i] == 0.0) continue;
         fr += r.out1[i] * r.r0[i];
         tr += r.r0[i];
       }
-      for (int j = 0; j < N; ++j) gc += r.out2[j] * r.c0[j];
+      for (int j = 0; j < N; ++j)
+        if (r.c0[j] != 0.0) gc += r.out2[j] * r.c0[j];
       r.loss = fr + gc - reg * tr;
       r.r0.clear();
       r.c0.clear();
@@ -716,6 +775,15 @@ class ServeEngine {
     auto c = std::make_unique<Cost>();
     c->M = M;
     c->N = N;
+    c->rowmin.assign(M, INFINITY);
+    c->colmin.assign(N, INFINITY);
+    for (long j = 0; j < N; ++j)
+      for (long i = 0; i < M; ++i) {
+        const double x = C[i + j * M];
+        RWIG_REQUIRE(std::isfinite(x), "the cost matrix must be finite");
+        c->rowmin[i] = std::min(c->rowmin[i], x);
+        c->colmin[j] = std::min(c->colmin[j], x);
+      }
     {
       std::lock_guard<std::mutex> lk(mu_);
       RWIG_REQUIRE(fatal_.empty(), "serving session failed: " + fatal_);
@@ -751,6 +819,25 @@ class ServeEngine {
     else
       RWIG_REQUIRE((int)r->in1.size() == c.M * r->S, "A must have M rows");
     RWIG_REQUIRE(r->S <= max_cols_, "request has more measures than a group holds");
+    // the shared Gibbs kernel exp(-C / reg) must not underflow on a row or
+    // column the request uses
+    auto too_small = [&](const char *what, long k, double cmin) {
+      throw Error(RWIG_ERR_INVALID,
+                  "reg = " + std::to_string(r->reg) + " is too small for this cost: " + what +
+                      " " + std::to_string(k) + " has min C / reg = " +
+                      std::to_string(cmin / r->reg) + " > " + std::to_string(kMaxCostOverReg) +
+                      ", where exp(-C / reg) underflows the low-precision Gibbs kernel; solve "
+                      "it with the log-domain form (rwig_sinkhorn_log, rwig_barycenter_log)");
+    };
+    for (long i = 0; i < c.M; ++i) {
+      bool used = false;
+      for (int s = 0; s < r->S && !used; ++s) used = r->in1[i + (long)s * c.M] > 0.0;
+      if (used && c.rowmin[i] / r->reg > kMaxCostOverReg) too_small("row", i, c.rowmin[i]);
+    }
+    for (long j = 0; j < c.N; ++j) {
+      const bool used = r->kind == kBarycenter || r->in2[j] > 0.0;
+      if (used && c.colmin[j] / r->reg > kMaxCostOverReg) too_small("column", j, c.colmin[j]);
+    }
     const size_t need = alone_bytes(c.M, c.N, r->S);
     RWIG_REQUIRE(cost_bytes_ + need <= budget_,
                  "request does not fit the session's HBM budget even alone");
@@ -1049,7 +1136,12 @@ class ServeEngine {
             std::lock_guard<std::mutex> lk(mu_);
             --running_;
           }
-          finish(r, true, "");
+          if (r->numeric)
+            finish(r, false,
+                   "the marginal error turned non-finite (a scaling or the Gibbs kernel left the "
+                   "floating-point range)");
+          else
+            finish(r, true, "");
         }
         cv_done_.notify_all();
       }
@@ -1057,7 +1149,7 @@ class ServeEngine {
       if (g->running.empty()) continue;
       g->fill_lists(1);
       g->run_phase(1);
-      for (Request *r : g->running) r->iter += absorb_every_;
+      for (Request *r : g->running) r->iter += std::min(absorb_every_, r->max_iter - r->iter);
     }
   }
 
@@ -1156,6 +1248,18 @@ ServeEngine *engine_of(void *h) {
   return static_cast<ServeEngine *>(h);
 }
 
+// A measure: finite non-negative entries with a positive total; zero entries
+// are off its support.
+void check_mass(const double *x, int n, const char *what) {
+  double t = 0.0;
+  for (int k = 0; k < n; ++k) {
+    RWIG_REQUIRE(std::isfinite(x[k]) && x[k] >= 0.0,
+                 std::string("entries of ") + what + " must be finite and non-negative");
+    t += x[k];
+  }
+  RWIG_REQUIRE(t > 0.0, std::string(what) + " must have positive mass");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1206,8 +1310,8 @@ int rwig_serve_submit_sinkhorn(void *session, int cost_id, double reg, const dou
   return guarded("rwig_serve_submit_sinkhorn", [&] {
     RWIG_REQUIRE(a && b && request_id && M > 0 && N > 0, "invalid argument");
     RWIG_REQUIRE(reg > 0.0 && max_iter >= 0, "reg must be positive, max_iter non-negative");
-    for (int i = 0; i < M; ++i) RWIG_REQUIRE(a[i] > 0.0, "entries of a must be positive");
-    for (int j = 0; j < N; ++j) RWIG_REQUIRE(b[j] > 0.0, "entries of b must be positive");
+    check_mass(a, M, "a");
+    check_mass(b, N, "b");
     auto r = std::make_unique<Request>();
     r->kind = kSinkhorn;
     r->cost = cost_id;
@@ -1227,8 +1331,7 @@ int rwig_serve_submit_barycenter(void *session, int cost_id, double reg, const d
   return guarded("rwig_serve_submit_barycenter", [&] {
     RWIG_REQUIRE(A && w && request_id && M > 0 && S >= 1, "invalid argument");
     RWIG_REQUIRE(reg > 0.0 && max_iter >= 0, "reg must be positive, max_iter non-negative");
-    for (long k = 0; k < (long)M * S; ++k)
-      RWIG_REQUIRE(A[k] > 0.0, "entries of A must be positive");
+    for (int s = 0; s < S; ++s) check_mass(A + (long)s * M, M, "each column of A");
     auto r = std::make_unique<Request>();
     r->kind = kBarycenter;
     r->cost = cost_id;
@@ -1250,12 +1353,13 @@ int rwig_serve_wait(void *session, long request_id, double timeout_s, int *state
 }
 
 int rwig_serve_result(void *session, long request_id, double *x, double *y, double *loss,
-                      int *iter, double *err, double *latency_ms) {
+                      int *iter, double *err, double *latency_ms, int *status) {
   int code = RWIG_OK;
   std::string msg;
   const int rc = guarded("rwig_serve_result", [&] {
     std::unique_ptr<Request> r = engine_of(session)->take(request_id);
-    if (r->state == kFailed) {
+    if (status) *status = r->status;
+    if (r->state == kFailed && !r->numeric) {
       code = RWIG_ERR_HIP;
       msg = "request failed: " + r->error;
       return;

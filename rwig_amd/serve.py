@@ -106,17 +106,22 @@ class ServeSession:
         return st.value
 
     def result(self, request_id):
-        """Waits for the request and returns its result dict (then forgotten)."""
+        """Waits for the request and returns its result dict (then forgotten).
+
+        return_status is the reference's (src/sinkhorn.cpp:88-95): 0 converged,
+        1 max_iter reached, 2 the solve broke down (non-finite error)."""
         self.wait(request_id)
         M, N = self._shapes[self._costs.pop(request_id)]
         kind = self._kinds.pop(request_id)
         x = np.zeros(M if kind == "sinkhorn" else N)
         y = np.zeros(N)
         loss, err, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        it = ctypes.c_int()
+        it, status = ctypes.c_int(), ctypes.c_int()
         check(lib().rwig_serve_result(self._h, int(request_id), _p(x), _p(y), ctypes.byref(loss),
-                                      ctypes.byref(it), ctypes.byref(err), ctypes.byref(lat)))
-        out = {"iter": it.value, "err": err.value, "latency_ms": lat.value}
+                                      ctypes.byref(it), ctypes.byref(err), ctypes.byref(lat),
+                                      ctypes.byref(status)))
+        out = {"iter": it.value, "err": err.value, "latency_ms": lat.value,
+               "return_status": status.value}
         if kind == "sinkhorn":
             out.update(f=x, g=y, loss=loss.value)
         else:

@@ -8,7 +8,10 @@ Bars:
 - a request's result is bit-identical alone and in a mix of other requests
   (same and other cost matrices, submitted from two threads while it runs);
 - the HBM footprint never exceeds the session's budget, and requests that do
-  not fit wait in the queue and still complete.
+  not fit wait in the queue and still complete;
+- each result carries the reference's return_status (0 converged, 1 max_iter,
+  2 breakdown); max_iter is honoured exactly; zero-mass entries are solved on
+  the support; a reg too small for the cost's range is refused.
 """
 import threading
 
@@ -175,14 +178,125 @@ def test_max_iter_and_invalid_input(clouds):
         a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
         r = s.result(s.submit_sinkhorn(cid, a, b, 0.01, max_iter=30, zero_tol=1e-12))
         assert r["iter"] == 30 and r["err"] > 1e-12
+        assert r["return_status"] == 1
         r0 = s.result(s.submit_sinkhorn(cid, a, b, 0.01, max_iter=0))
-        assert r0["iter"] == 0
-        bad = a.copy()
-        bad[3] = 0.0
-        with pytest.raises(RwigError):
-            s.submit_sinkhorn(cid, bad, b, 0.1)
+        assert r0["iter"] == 0 and r0["return_status"] == 1
+        for v in (-1e-3, np.nan, np.inf):
+            bad = a.copy()
+            bad[3] = v
+            with pytest.raises(RwigError):
+                s.submit_sinkhorn(cid, bad, b, 0.1)
+        with pytest.raises(RwigError, match="positive mass"):
+            s.submit_sinkhorn(cid, np.zeros(n), b, 0.1)
         with pytest.raises(RwigError):
             s.submit_sinkhorn(cid + 5, a, b, 0.1)
+
+
+@pytest.mark.parametrize("kind", ["sinkhorn", "barycenter"])
+def test_max_iter_not_a_multiple_of_absorb_every(clouds, kind):
+    # the last cycle runs only the iterations left: 50 = 20 + 20 + 10
+    rng = np.random.default_rng(7)
+    n = clouds[0].shape[0]
+    with ServeSession(hbm_budget_bytes=1 << 30, absorb_every=20) as s:
+        cid = s.register_cost(clouds[0])
+        if kind == "sinkhorn":
+            a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+            r = s.result(s.submit_sinkhorn(cid, a, b, 0.01, max_iter=50, zero_tol=1e-300))
+            # the reported error is that of the plan after exactly 50 iterations
+            assert abs(marginal_err(clouds[0], 0.01, r["f"], r["g"], a, b) - r["err"]) <= 1e-9
+        else:
+            A = rng.dirichlet(np.ones(n), size=3).T.copy()
+            r = s.result(s.submit_barycenter(cid, A, rng.dirichlet(np.ones(3)), 0.05,
+                                             max_iter=50, zero_tol=1e-300))
+        assert r["iter"] == 50 and r["return_status"] == 1
+        ok = s.result(s.submit_sinkhorn(cid, rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n)),
+                                        0.1))
+        assert ok["return_status"] == 0 and ok["err"] <= 1e-6
+
+
+def test_reg_too_small_for_the_cost_is_refused(clouds):
+    # every entry of a shifted cost is >= 0.9: at reg 0.01 exp(-C/reg) <= e^-90
+    # underflows the BF16 Gibbs kernel, so the request is refused up front
+    n = clouds[0].shape[0]
+    u = np.full(n, 1.0 / n)
+    with ServeSession(hbm_budget_bytes=1 << 30) as s:
+        cid = s.register_cost(clouds[0] + 0.9)
+        with pytest.raises(RwigError, match="too small for this cost"):
+            s.submit_sinkhorn(cid, u, u, 0.01)
+        with pytest.raises(RwigError, match="too small for this cost"):
+            s.submit_barycenter(cid, np.stack([u, u], 1), np.array([0.5, 0.5]), 0.01)
+        r = s.result(s.submit_sinkhorn(cid, u, u, 0.1))  # min C / reg = 9: served
+        assert r["return_status"] == 0 and r["err"] <= 1e-6
+
+
+def test_zero_mass_sinkhorn_is_solved_on_the_support(clouds):
+    # the reference reduces the problem to the supports of a and b
+    # (src/sinkhorn_impl_log.cpp:225-294): potentials -inf off them
+    rng = np.random.default_rng(8)
+    C = clouds[1]
+    n = C.shape[0]
+    a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(n))
+    a[rng.choice(n, n // 3, replace=False)] = 0.0
+    b[rng.choice(n, n // 2, replace=False)] = 0.0
+    a /= a.sum()
+    b /= b.sum()
+    with ServeSession(hbm_budget_bytes=1 << 30) as s:
+        cid = s.register_cost(C)
+        for eps in (0.1, 0.01):
+            r = s.result(s.submit_sinkhorn(cid, a, b, eps))
+            assert r["return_status"] == 0 and r["err"] <= 1e-6
+            assert np.all(np.isneginf(r["f"][a == 0])) and np.all(np.isfinite(r["f"][a > 0]))
+            assert np.all(np.isneginf(r["g"][b == 0])) and np.all(np.isfinite(r["g"][b > 0]))
+            ia, ib = a > 0, b > 0
+            e_host = marginal_err(C[np.ix_(ia, ib)], eps, r["f"][ia], r["g"][ib], a[ia], b[ib])
+            assert abs(e_host - r["err"]) <= 1e-9
+            o = O.sinkhorn_log(a, b, C, eps, maxiter=5000, zerotol=1e-12)
+            assert np.all(np.isneginf(o["f"][a == 0]))
+            assert abs(r["loss"] - o["loss"]) / abs(o["loss"]) < TOL_LOSS
+
+
+def test_zero_mass_barycenter_inputs(clouds):
+    # rows where A_s is zero keep a zero scaling, as U = A / KV gives in the
+    # reference's parallel barycenter (src/barycenter_impl_parallel.cpp:102-150)
+    rng = np.random.default_rng(9)
+    C = clouds[0]
+    n, S = C.shape[0], 3
+    A = rng.dirichlet(np.ones(n), size=S).T.copy()
+    for s_ in range(S):
+        A[rng.choice(n, n // 4, replace=False), s_] = 0.0
+    A /= A.sum(0)
+    w = rng.dirichlet(np.ones(S))
+    with ServeSession(hbm_budget_bytes=1 << 30) as s:
+        cid = s.register_cost(C)
+        r = s.result(s.submit_barycenter(cid, A, w, 0.1))
+    o = O.barycenter_parallel(A, C, w, None, 0.1, maxiter=20000, zerotol=1e-13)
+    assert r["return_status"] == 0 and r["err"] <= 1e-6
+    assert np.abs(r["b"] - o["b"]).sum() < TOL_BARY
+
+
+def test_breakdown_ends_as_failed_with_status_2():
+    # supports far apart on a line: every cost entry between them is >= 0.27,
+    # so at reg 0.002 the BF16 Gibbs kernel is zero on all of them although
+    # each row and column has a zero-cost entry (the guard passes). The
+    # scalings overflow, the error turns non-finite, and the request ends
+    # failed with the reference's status 2 instead of "converged"
+    from rwig_amd.serve import FAILED
+    x = np.linspace(0.0, 1.0, 64)
+    C = (x[:, None] - x[None]) ** 2
+    a = np.where(x <= 0.24, 1.0, 0.0)
+    b = np.where(x >= 0.76, 1.0, 0.0)
+    a /= a.sum()
+    b /= b.sum()
+    with ServeSession(hbm_budget_bytes=1 << 30) as s:
+        cid = s.register_cost(C)
+        rid = s.submit_sinkhorn(cid, a, b, 0.002, max_iter=200)
+        assert s.wait(rid) == FAILED
+        r = s.result(rid)
+        assert r["return_status"] == 2 and not np.isfinite(r["err"])
+        ok = s.result(s.submit_sinkhorn(cid, np.full(64, 1 / 64), np.full(64, 1 / 64), 0.05))
+        assert ok["return_status"] == 0
+        st = s.stats()
+    assert st["failed"] == 1 and st["completed"] == 1
 
 
 def test_graph_replay_equals_eager(clouds):

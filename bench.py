@@ -110,13 +110,19 @@ def gemm_only():
 
 def c3_leg(log):
     """Barycenter with unrolled gradient (64 x 64 grid, S = 8, eps 0.004,
-    L = 200, checkpoints every 20): one forward + adjoint in FP32 and FP64."""
+    L = 200, checkpoints every 20): one forward + adjoint in FP32, in the
+    mixed form (FP32 exponentials, FP64 arguments) and in FP64."""
     import rwig_amd
     from rwig_amd.workloads import c3
     A, C, w, bext, eps, L = c3()
     out = {"workload": "log barycenter + checkpointed adjoint, 64x64 grid (n=4096), S=8, "
-                       "eps=0.004, L=200, checkpoint every 20, seed unnormalised"}
-    for prec in ("fp32", "fp64"):
+                       "eps=0.004, L=200, checkpoint every 20, seed unnormalised",
+           "precisions": {"fp32": "FP32 cost, iterates and passes",
+                          "fp32exp": "FP64 cost, iterates and exponent arguments, FP32 "
+                                     "exponentials (within 1e-5 of FP64 on b, grad_w and grad_A "
+                                     "where A >= 1e-12)",
+                          "fp64": "FP64 throughout"}}
+    for prec in ("fp32", "fp32exp", "fp64"):
         kw = dict(reg=eps, max_iter=L, zero_tol=0.0, with_grad=True, seed="unnormalised",
                   checkpoint_every=20, precision=prec)
         rwig_amd.barycenter_log_ck(A, C, w, bext, **kw)  # warm-up

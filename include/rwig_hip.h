@@ -91,8 +91,11 @@ int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A,
  * max_iter/checkpoint_every + checkpoint_every + 2 slices of (M + N) x S
  * instead of max_iter + 1.
  *   precision 0: FP64 throughout (equal to rwig_barycenter_log);
- *   precision 1: FP32 iterates and soft-min passes, FP64 gradient
- *                accumulation, loss and adjoint seed.
+ *   precision 1: FP32 cost, iterates and soft-min passes, FP64 gradient
+ *                accumulation, loss and adjoint seed;
+ *   precision 2: FP64 cost, iterates and exponent arguments, the passes'
+ *                exponentials in FP32 (the FP32 rounding of C alone moves
+ *                every exponent by up to 2^-24 max|C| / reg).
  *   seed_mode 0: the reference's adjoint seed 2 (b - b_ext) .* b, b
  *                normalised (src/barycenter_impl_log.cpp:323-325);
  *   seed_mode 1: exact gradient of ||b_L - b_ext||^2, b_L = exp(logb_L)

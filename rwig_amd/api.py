@@ -163,14 +163,16 @@ _SEEDS = {"reference": 0, "unnormalised": 1, "normalised": 2}
 
 def barycenter_log_ck(A, C, w, b_ext=None, reg=0.1, with_grad=False, max_iter=1000,
                       zero_tol=1e-6, checkpoint_every=20, seed="reference", precision="fp64"):
-    """Log barycenter with a checkpointed adjoint, in FP64 or FP32.
+    """Log barycenter with a checkpointed adjoint.
 
     Same iteration and outputs as barycenter(method="log")
     (src/barycenter_impl_log.cpp:189-382). seed picks the adjoint's starting
     gradient: "reference" (2 (b - b_ext) .* b, b normalised, as the
     reference), "unnormalised" (exact gradient of ||exp(logb_L) - b_ext||^2)
     or "normalised" (exact gradient of ||b - b_ext||^2 through the
-    normalisation).
+    normalisation). precision: "fp64"; "fp32" (C, iterates and soft-min
+    passes in FP32); "fp32exp" (FP64 C, iterates and exponent arguments, the
+    passes' exponentials in FP32).
     """
     A = _f64(A)
     C = _f64(C)
@@ -181,7 +183,7 @@ def barycenter_log_ck(A, C, w, b_ext=None, reg=0.1, with_grad=False, max_iter=10
         raise ValueError("you must supply the `b_ext` with `with_grad = TRUE`!")
     if seed not in _SEEDS:
         raise ValueError("seed must be one of " + ", ".join(_SEEDS))
-    prec = {"fp64": 0, "fp32": 1}[precision]
+    prec = {"fp64": 0, "fp32": 1, "fp32exp": 2}[precision]
     M, S = A.shape
     N = C.shape[1]
     if w.size != S:

@@ -7,6 +7,14 @@
 // (src/barycenter_impl_parallel.cpp:60-151), which is what lets the FP32 form
 // run at regularisations where exp(-C/reg) underflows FP32.
 //
+// Precision: FP64; FP32 (C, iterates and passes in FP32); or the mixed form,
+// FP64 C, iterates and exponent arguments with the exponentials of the passes
+// in FP32. FP32 storage of C alone perturbs every exponent by up to
+// 2^-24 max|C| / reg (1.5e-5 at the 64 x 64 grid's reg 0.004), which bounds
+// the FP32 gradient's accuracy near 1e-5; the mixed form forms each argument
+// in FP64 and rounds only that, at about the FP32 form's cost (FP64 add and
+// FMA run at the FP32 rate on CDNA4; the FP64 exponential is the slow part).
+//
 // Memory: the reference keeps all L+1 iterates for the adjoint
 // (src/barycenter_impl_log.cpp:197-209). Here the forward stores
 // (F, G, row soft-min) only at iterations 0, ck, 2 ck, ...; the adjoint walks
@@ -73,8 +81,10 @@ struct BaryCkResult {
 // Device pointers of type T: C (M x N), w (S), bext (N) in; b (N), F (M x S),
 // G (N x S) out. FP64: A (M x S) in (its log and the division of the
 // gradient by A stay in FP64, so FP32 runs take entries of A below FP32's
-// normal range); gA (M x S), gw (S), loss out.
-template <typename T>
+// normal range); gA (M x S), gw (S), loss out. X is the type the soft-min and
+// weighted-sum passes evaluate their exponentials in (X = float, T = double:
+// FP64 arguments, FP32 exponentials).
+template <typename T, typename X>
 BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, const T *w,
                                const T *bext, int M, int N, int S, double reg, bool wg,
                                int maxiter, double tol, int ck, int seed, T *bout, T *Fout,
@@ -114,7 +124,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
     h.active = (0 < maxiter && h.err >= tol) ? 1 : 0;
     h2d(ls, &h, 1, st);
   }
-  lse_row<T>(
+  lse_row<T, X>(
       st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
       [=] __device__(long k, T r) {
         KV[k] = r;
@@ -144,7 +154,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         gate);
     // KTU = column soft-min of R(F, G); logb = -(G + KTU) w / reg;
     // G += reg logb 1^T + KTU
-    lse_col_t<T>(
+    lse_col_t<T, X>(
         st, CT, M, N, N, F, M, G, N, S, reg, cpm, cps,
         [=] __device__(int j, const T *r) {
           T acc = T(0);
@@ -166,7 +176,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         },
         gate);
     // KV = row soft-min of R(F, G)
-    lse_row<T>(
+    lse_row<T, X>(
         st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
         [=] __device__(long k, T r) {
           KV[k] = r;
@@ -290,7 +300,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         const T *KVl = hKV + q * MS, *KTUl = hKTU + q * NS;
         if (l != L) {
           // Gbar(:,s) = -X_s^T Fbar(:,s); logbbar = reg * rowsum(Gbar)
-          wcol_t<T>(
+          wcol_t<T, X>(
               st, CT, M, N, N, Fl, M, Gl, N, S, reg, KVl, Fbar, cpart,
               [=] __device__(int j, const T *r) {
                 T acc = T(0);
@@ -315,7 +325,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         });
         // computed relative to each row's A (Q = Fbar / A, rows shifted by
         // reg log A): rows of A far below the FP32 range keep their gradient
-        wrow<T>(
+        wrow<T, X>(
             st, C, M, N, M, Fl, M, Glm1, N, S, reg, KTUl, Y, part,
             [=] __device__(long k, T q) {
               Fbar[k] = (T)((double)q * A[k]);
@@ -333,7 +343,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
   return {L, hs.err};
 }
 
-template <typename T>
+template <typename T, typename X>
 BaryCkResult run_ck(hipStream_t st, double *Fh, double *Gh, double *bh, double *gAh, double *gwh,
                     double *lossh, const double *A, const double *w, const double *C,
                     const double *bext, int M, int N, int S, double reg, bool wg, int maxiter,
@@ -355,7 +365,7 @@ BaryCkResult run_ck(hipStream_t st, double *Fh, double *Gh, double *bh, double *
   const T *dbext = wg ? upload(bext, N) : nullptr;
   T *db = ar.make<T>(N), *dF = ar.make<T>(MS), *dG = ar.make<T>(NS);
   double *dgA = ar.make<double>(MS), *dgw = ar.make<double>(S), *dloss = ar.make<double>(1);
-  const BaryCkResult res = barycenter_log_ck<T>(st, dA, dC, dw, dbext, M, N, S, reg, wg, maxiter,
+  const BaryCkResult res = barycenter_log_ck<T, X>(st, dA, dC, dw, dbext, M, N, S, reg, wg, maxiter,
                                                 tol, ck, seed, db, dF, dG, dgA, dgw, dloss);
   auto download = [&](double *h, const T *d, long n) {
     if (!h) return;
@@ -397,15 +407,19 @@ extern "C" int rwig_barycenter_log_ck(double *F, double *G, double *b, double *g
     RWIG_REQUIRE(!withgrad || b_ext, "b_ext is required with withgrad");
     RWIG_REQUIRE(checkpoint_every >= 1, "checkpoint_every must be positive");
     RWIG_REQUIRE(seed_mode >= 0 && seed_mode <= 2, "seed_mode must be 0, 1 or 2");
-    RWIG_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32)");
+    RWIG_REQUIRE(precision >= 0 && precision <= 2,
+                 "precision must be 0 (FP64), 1 (FP32) or 2 (FP64 arguments, FP32 exponentials)");
     ensure_device();
     Stream s;
-    const BaryCkResult res =
-        precision == 0
-            ? run_ck<double>(s.get(), F, G, b, grad_A, grad_w, loss, A, w, C, b_ext, M, N, S,
-                             reg, withgrad != 0, max_iter, zero_tol, checkpoint_every, seed_mode)
-            : run_ck<float>(s.get(), F, G, b, grad_A, grad_w, loss, A, w, C, b_ext, M, N, S,
-                            reg, withgrad != 0, max_iter, zero_tol, checkpoint_every, seed_mode);
+    auto run = [&](auto tag_t, auto tag_x) {
+      using T = decltype(tag_t);
+      using X = decltype(tag_x);
+      return run_ck<T, X>(s.get(), F, G, b, grad_A, grad_w, loss, A, w, C, b_ext, M, N, S, reg,
+                          withgrad != 0, max_iter, zero_tol, checkpoint_every, seed_mode);
+    };
+    const BaryCkResult res = precision == 0   ? run(double(), double())
+                             : precision == 1 ? run(float(), float())
+                                              : run(double(), float());
     if (iter_out) *iter_out = res.iter;
     if (err_out) *err_out = res.err;
     if (return_status_out) *return_status_out = status_of(res.err, zero_tol, res.iter, max_iter);

@@ -213,10 +213,33 @@ constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kLn2 = 0.69314718055994530942;
 __device__ __forceinline__ double sm_exp2(double x) { return exp2(x); }
 __device__ __forceinline__ float sm_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+// exp2 of an argument held in T, evaluated in X: X = float with T = double is
+// the mixed form (FP64 arguments, the hardware FP32 exponential). The
+// argument is formed in FP64 and only then rounded, so the weight's relative
+// error is about 2^-24 times the argument, which is <= 0 and small for the
+// weights that count; an FP32 argument would carry the rounding of C and of
+// the shifts instead (2^-24 max|C| / reg relative: 1.5e-5 at max C 1 and
+// reg 0.004).
+template <typename X, typename T> __device__ __forceinline__ T ex2(T x) {
+  return (T)sm_exp2((X)x);
+}
+// The same over T's whole exponent range: in the mixed form x = k + r with
+// k = floor(x), 2^r in FP32 on [0, 1) and 2^k applied exactly in FP64. For
+// the weighted sums, whose fixed shifts can leave every term of a row far
+// below FP32's range (2^-126) while FP64 holds them.
+template <typename X, typename T> __device__ __forceinline__ T ex2_wide(T x) {
+  if constexpr (sizeof(X) < sizeof(T)) {
+    const T k = fmin(fmax(floor(x), T(-1100)), T(1100));
+    return ldexp((T)sm_exp2((X)(x - k)), (int)k);
+  } else {
+    return (T)sm_exp2((X)x);
+  }
+}
 
 // Online log-sum-exp state in base-2 units: running minimum m and
-// t = sum exp2(-(r - m)), rescaled when the minimum drops.
-template <typename T> struct SoftMin {
+// t = sum exp2(-(r - m)), rescaled when the minimum drops. X is the type the
+// exponentials are evaluated in.
+template <typename T, typename X = T> struct SoftMin {
   T m, t;
   __device__ void init() {
     m = T(INFINITY);
@@ -224,15 +247,15 @@ template <typename T> struct SoftMin {
   }
   __device__ void add(T r) {
     const T d = r - m;
-    const T e = sm_exp2(-fabs(d));
+    const T e = ex2<X>(-fabs(d));
     const bool lt = d < T(0);
     t = lt ? fma(t, e, T(1)) : t + e;
     m = lt ? r : m;
   }
   __device__ void merge(T m2, T t2) {
     const T mn = fmin(m, m2);
-    const T a = (m == mn) ? t : t * sm_exp2(mn - m);
-    const T b = (m2 == mn) ? t2 : t2 * sm_exp2(mn - m2);
+    const T a = (m == mn) ? t : t * ex2<X>(mn - m);
+    const T b = (m2 == mn) ? t2 : t2 * ex2<X>(mn - m2);
     m = mn;
     t = a + b;
   }
@@ -240,10 +263,10 @@ template <typename T> struct SoftMin {
   // the new minimum, then one exponential per value (no compare or select)
   template <int G> __device__ void add_group(T mg, const T (&r)[G]) {
     const T mn = fmin(m, mg);
-    if (m != mn) t *= sm_exp2(mn - m);
+    if (m != mn) t *= ex2<X>(mn - m);
     m = mn;
 #pragma unroll
-    for (int u = 0; u < G; ++u) t += sm_exp2(m - r[u]);
+    for (int u = 0; u < G; ++u) t += ex2<X>(m - r[u]);
   }
   // the soft-min in the caller's units: (m - log2 t) / sc
   __device__ T value(T back) const { return (m - log2(t)) * back; }
@@ -396,14 +419,18 @@ __device__ __forceinline__ void stage_cols(const T *__restrict__ src, long ld, i
 // the online update (row pass at the C3 shape 47 -> 37 us). FP64 keeps the
 // online update: its exponential is a long instruction sequence, which the
 // group's extra rescale and unrolled code make slower, not faster.
-template <typename T, int SM> struct RowGroup {
+template <typename T, int SM, typename X = T> struct RowGroup {
   // columns per group: their values and one slice's terms stay in registers;
-  // the rescale costs one exponential per group and slice
-  static constexpr int G = sizeof(T) == 4 ? (SM <= 8 ? 16 : 8) : kColUnroll;
-  static constexpr bool two_step = sizeof(T) == 4;
+  // the rescale costs one exponential per group and slice. The mixed form
+  // (FP64 values, FP32 exponentials) groups as FP32 does, with fewer columns
+  // for its 8-byte registers (no scratch at any S).
+  static constexpr int G =
+      sizeof(X) == 4 ? (sizeof(T) == 4 ? (SM <= 8 ? 16 : 8) : (SM <= 4 ? 8 : (SM <= 8 ? 4 : 2)))
+                     : kColUnroll;
+  static constexpr bool two_step = sizeof(X) == 4;
 };
 
-template <typename T, int SM>
+template <typename T, int SM, typename X = T>
 __global__ void __launch_bounds__(kTB)
 k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
                long ldf, const T *__restrict__ G, long ldg, int S, T reg, int cw,
@@ -416,10 +443,10 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
   const int j0 = c * cw, j1 = min(N, j0 + cw);
   const T sc = T(kLog2e / (double)reg);
   const bool row = i < M;
-  SoftMin<T> st[SM];
+  SoftMin<T, X> st[SM];
 #pragma unroll
   for (int s = 0; s < SM; ++s) st[s].init();
-  constexpr int GR = RowGroup<T, SM>::G;
+  constexpr int GR = RowGroup<T, SM, X>::G;
   __shared__ T gs[kColTile * SM];
   for (int t0 = j0; t0 < j1; t0 += kColTile) {
     const int nc = min(kColTile, j1 - t0);
@@ -433,7 +460,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
         T x[GR];
 #pragma unroll
         for (int u = 0; u < GR; ++u) x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
-        if constexpr (RowGroup<T, SM>::two_step) {
+        if constexpr (RowGroup<T, SM, X>::two_step) {
 #pragma unroll
           for (int s = 0; s < SM; ++s) {
             T r[GR];
@@ -464,22 +491,25 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
       }
     }
   }
-  // merge the wavefronts' states, one slice at a time through LDS
+  // merge the wavefronts' states, one slice at a time through LDS (S is
+  // uniform over the workgroup; a guard rather than a loop exit keeps the
+  // loop fully unrolled, so st[] stays in registers)
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
-    if (s >= S) break;
-    __syncthreads();
-    shm[w][lane] = st[s].m;
-    sht[w][lane] = st[s].t;
-    __syncthreads();
-    if (w == 0 && row) {
-      SoftMin<T> a;
-      a.m = shm[0][lane];
-      a.t = sht[0][lane];
-      for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane]);
-      const long o = ((long)c * S + s) * M + i;
-      pm[o] = a.m - F[i + (long)s * ldf] * sc;
-      ps[o] = a.t;
+    if (s < S) {
+      __syncthreads();
+      shm[w][lane] = st[s].m;
+      sht[w][lane] = st[s].t;
+      __syncthreads();
+      if (w == 0 && row) {
+        SoftMin<T, X> a;
+        a.m = shm[0][lane];
+        a.t = sht[0][lane];
+        for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][lane], sht[v][lane]);
+        const long o = ((long)c * S + s) * M + i;
+        pm[o] = a.m - F[i + (long)s * ldf] * sc;
+        ps[o] = a.t;
+      }
     }
   }
 }
@@ -511,14 +541,14 @@ k_lse_row_merge(const T *__restrict__ pm, const T *__restrict__ ps, int nchunk, 
 
 // Soft-min over columns j of R_s(i,j) for every row i and slice s:
 // op(k = s*M + i, value).
-template <typename T, class Op>
+template <typename T, typename X = T, class Op>
 void lse_row(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf,
              const T *G, long ldg, int S, double reg, T *pm, T *ps, Op op,
              const LoopState *ls) {
   const Chunking ch = row_chunking(M, N);
   const dim3 grid((M + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
   RWIG_DISPATCH_S(S, SM,
-                  hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, C, M, N,
+                  hipLaunchKernelGGL((k_lse_row_part<T, SM, X>), grid, dim3(kTB), 0, st, C, M, N,
                                      ldc, F, ldf, G, ldg, S, (T)reg, ch.cw, pm, ps, ls));
   check_launch();
   const long len = (long)S * M;
@@ -618,7 +648,7 @@ void wcol(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long l
 // rowsh (optional, M x S) scales row i by exp(-rowsh/reg) inside the
 // exponent, so rows whose result is tiny can be computed relative to their
 // own scale without underflow. Layout as k_lse_row_part.
-template <typename T, int SM>
+template <typename T, int SM, typename X = T>
 __global__ void __launch_bounds__(kTB)
 k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F, long ldf,
             const T *__restrict__ G, long ldg, int S, T reg, const T *__restrict__ cshift,
@@ -663,27 +693,29 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
           const int q = (jj + u * kWavesPerBlock) * SM;
           const T xs = x[u] * sc;
 #pragma unroll
-          for (int s = 0; s < SM; ++s) acc[s] += sm_exp2(-((xs - f[s]) - gh[q + s])) * ys[q + s];
+          for (int s = 0; s < SM; ++s)
+            acc[s] += ex2_wide<X>(-((xs - f[s]) - gh[q + s])) * ys[q + s];
         }
       }
       for (; jj < nc; jj += kWavesPerBlock) {
         const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
         const T *g = gh + jj * SM, *yv = ys + jj * SM;
 #pragma unroll
-        for (int s = 0; s < SM; ++s) acc[s] += sm_exp2(-((xs - f[s]) - g[s])) * yv[s];
+        for (int s = 0; s < SM; ++s) acc[s] += ex2_wide<X>(-((xs - f[s]) - g[s])) * yv[s];
       }
     }
   }
 #pragma unroll
   for (int s = 0; s < SM; ++s) {
-    if (s >= S) break;
-    __syncthreads();
-    sha[w][lane] = acc[s];
-    __syncthreads();
-    if (w == 0 && row) {
-      T v = sha[0][lane];
-      for (int u = 1; u < kWavesPerBlock; ++u) v += sha[u][lane];
-      part[((long)c * S + s) * M + i] = v;
+    if (s < S) {
+      __syncthreads();
+      sha[w][lane] = acc[s];
+      __syncthreads();
+      if (w == 0 && row) {
+        T v = sha[0][lane];
+        for (int u = 1; u < kWavesPerBlock; ++u) v += sha[u][lane];
+        part[((long)c * S + s) * M + i] = v;
+      }
     }
   }
 }
@@ -692,14 +724,14 @@ inline size_t wrow_part_size(int M, int N, int S) {
   return (size_t)row_chunking(M, N).nchunk * S * M;
 }
 
-template <typename T, class Op>
+template <typename T, typename X = T, class Op>
 void wrow(hipStream_t st, const T *C, int M, int N, long ldc, const T *F, long ldf, const T *G,
           long ldg, int S, double reg, const T *cshift, const T *y, T *part, Op op,
           const LoopState *ls, const T *rowsh = nullptr) {
   const Chunking ch = row_chunking(M, N);
   const dim3 grid((M + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
   RWIG_DISPATCH_S(S, SM,
-                  hipLaunchKernelGGL((k_wrow_part<T, SM>), grid, dim3(kTB), 0, st, C, M, N, ldc,
+                  hipLaunchKernelGGL((k_wrow_part<T, SM, X>), grid, dim3(kTB), 0, st, C, M, N, ldc,
                                      F, ldf, G, ldg, S, (T)reg, cshift, y, rowsh, ch.cw, part,
                                      ls));
   check_launch();
@@ -797,15 +829,15 @@ inline size_t col_t_part_size(int M, int N, int S) {
 }
 
 // column soft-min of C (as lse_col) from CT; pm, ps: col_t_part_size each
-template <typename T, class Op>
+template <typename T, typename X = T, class Op>
 void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F, long ldf,
                const T *G, long ldg, int S, double reg, T *pm, T *ps, Op op,
                const LoopState *ls) {
   const Chunking ch = row_chunking(N, M);
   const dim3 grid((N + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
   RWIG_DISPATCH_S(S, SM, {
-    hipLaunchKernelGGL((k_lse_row_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg,
-                       F, ldf, S, (T)reg, ch.cw, pm, ps, ls);
+    hipLaunchKernelGGL((k_lse_row_part<T, SM, X>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G,
+                       ldg, F, ldf, S, (T)reg, ch.cw, pm, ps, ls);
     check_launch();
     // merged values into the room after the partials, then the column form
     T *mv = pm + (size_t)ch.nchunk * S * N;
@@ -822,15 +854,15 @@ void lse_col_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F,
 }
 
 // column weighted sums of C (as wcol) from CT; part: col_t_part_size
-template <typename T, class Op>
+template <typename T, typename X = T, class Op>
 void wcol_t(hipStream_t st, const T *CT, int M, int N, long ldct, const T *F, long ldf,
             const T *G, long ldg, int S, double reg, const T *rshift, const T *x, T *part, Op op,
             const LoopState *ls) {
   const Chunking ch = row_chunking(N, M);
   const dim3 grid((N + kRowsPerBlock - 1) / kRowsPerBlock, ch.nchunk);
   RWIG_DISPATCH_S(S, SM, {
-    hipLaunchKernelGGL((k_wrow_part<T, SM>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg, F,
-                       ldf, S, (T)reg, rshift, x, (const T *)nullptr, ch.cw, part, ls);
+    hipLaunchKernelGGL((k_wrow_part<T, SM, X>), grid, dim3(kTB), 0, st, CT, N, M, ldct, G, ldg,
+                       F, ldf, S, (T)reg, rshift, x, (const T *)nullptr, ch.cw, part, ls);
     check_launch();
     hipLaunchKernelGGL((k_sum_cols<T, SM, Op>), dim3((N + kWave / SM - 1) / (kWave / SM)),
                        dim3(kMergeTB), 0, st, part, ch.nchunk, N, S, op, ls);

@@ -379,8 +379,10 @@ int rwig_serve_wait(void *session, long request_id, double timeout_s,
                     int *state);
 /* Results of a finished request, which is then forgotten. Sinkhorn: x = f
  * (M), y = g (N) potentials, loss = sum C.*P + reg sum P.*(log P - 1).
- * Barycenter: x = b (N, normalised), y unused. latency_ms: submit to
- * completion. status (may be null): the reference's return_status
+ * Barycenter: x = b (N, normalised); y (may be null, M*S + N*S values) = the
+ * potentials F (M x S) then G (N x S) of the absorbed plans
+ * P_s = exp((F_s 1^T + 1 G_s^T - C) / reg), as the reference's log barycenter
+ * returns F and G. latency_ms: submit to completion. status (may be null): the reference's return_status
  * (src/sinkhorn.cpp:88-95): 0 err <= zero_tol, 1 max_iter reached, 2 the
  * solve broke down (non-finite error; the call still returns RWIG_OK and the
  * outputs as they stood). A request failed by the engine itself (HIP error,

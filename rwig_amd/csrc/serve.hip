@@ -716,8 +716,13 @@ struct Group {
       d2h(r.r0.data(), r0.p<double>() + c * M, (size_t)M, st);
       d2h(r.c0.data(), c0.p<double>() + c * N, (size_t)N, st);
     } else {
+      // b, and the potentials F (M x S) then G (N x S) of the absorbed plans
       r.out1.resize(N);
       d2h(r.out1.data(), bcol.p<double>() + c * N, (size_t)N, st);
+      const size_t ms = (size_t)M * r.S, ns = (size_t)N * r.S;
+      r.out2.resize(ms + ns);
+      d2h(r.out2.data(), F.p<double>() + c * M, ms, st);
+      d2h(r.out2.data() + ms, G.p<double>() + c * N, ns, st);
     }
   }
 
@@ -1365,7 +1370,7 @@ int rwig_serve_result(void *session, long request_id, double *x, double *y, doub
       return;
     }
     if (x) std::copy(r->out1.begin(), r->out1.end(), x);
-    if (y && r->kind == kSinkhorn) std::copy(r->out2.begin(), r->out2.end(), y);
+    if (y) std::copy(r->out2.begin(), r->out2.end(), y);
     if (loss) *loss = r->loss;
     if (iter) *iter = r->iter;
     if (err) *err = r->err;

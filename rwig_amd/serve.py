@@ -85,7 +85,7 @@ class ServeSession:
                                                _p(b), b.size, int(max_iter), float(zero_tol),
                                                ctypes.byref(rid)))
         self._costs[rid.value] = cost_id
-        self._kinds[rid.value] = "sinkhorn"
+        self._kinds[rid.value] = ("sinkhorn", 1)
         return rid.value
 
     def submit_barycenter(self, cost_id, A, w, reg, max_iter=1000, zero_tol=1e-6):
@@ -96,7 +96,7 @@ class ServeSession:
                                                  A.shape[0], A.shape[1], _p(w), int(max_iter),
                                                  float(zero_tol), ctypes.byref(rid)))
         self._costs[rid.value] = cost_id
-        self._kinds[rid.value] = "barycenter"
+        self._kinds[rid.value] = ("barycenter", A.shape[1])
         return rid.value
 
     def wait(self, request_id, timeout_s=-1.0):
@@ -112,9 +112,9 @@ class ServeSession:
         1 max_iter reached, 2 the solve broke down (non-finite error)."""
         self.wait(request_id)
         M, N = self._shapes[self._costs.pop(request_id)]
-        kind = self._kinds.pop(request_id)
+        kind, S = self._kinds.pop(request_id)
         x = np.zeros(M if kind == "sinkhorn" else N)
-        y = np.zeros(N)
+        y = np.zeros(N if kind == "sinkhorn" else (M + N) * S)
         loss, err, lat = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         it, status = ctypes.c_int(), ctypes.c_int()
         check(lib().rwig_serve_result(self._h, int(request_id), _p(x), _p(y), ctypes.byref(loss),
@@ -126,6 +126,8 @@ class ServeSession:
             out.update(f=x, g=y, loss=loss.value)
         else:
             out["b"] = x
+            out["F"] = y[:M * S].reshape((M, S), order="F")
+            out["G"] = y[M * S:].reshape((N, S), order="F")
         return out
 
     def stats(self):

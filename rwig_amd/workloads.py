@@ -73,15 +73,25 @@ def c3(grid=64, S=8, seed=1003):
     return np.asfortranarray(A), np.asfortranarray(C), w, b_ext, 0.004, 200
 
 
-def c5_clouds(n_clouds=32, seed=1005):
+def c5_sizes(n_clouds=32, seed=1005):
+    """The point counts of c5_clouds (same seed, no matrices built)."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(64), np.log(8192), n_clouds))).astype(int)
+
+
+def c5_clouds(n_clouds=32, seed=1005, only=None):
     """Serving mix, cost side: n_clouds point clouds of n_k points uniform in
     [0,1]^2, n_k = round(exp(U[ln 64, ln 8192])), squared Euclidean cost.
-    Returns the list of cost matrices (n_k x n_k, Fortran order)."""
+    Returns the list of cost matrices (n_k x n_k, Fortran order); with only
+    (a set of indices) the other entries are None, the random stream the same."""
     rng = np.random.default_rng(seed)
     sizes = np.round(np.exp(rng.uniform(np.log(64), np.log(8192), n_clouds))).astype(int)
     out = []
-    for n in sizes:
+    for k, n in enumerate(sizes):
         X = rng.random((int(n), 2))
+        if only is not None and k not in only:
+            out.append(None)
+            continue
         sq = (X * X).sum(1)
         C = np.maximum(sq[:, None] + sq[None, :] - 2.0 * X @ X.T, 0.0)
         np.fill_diagonal(C, 0.0)

@@ -245,13 +245,47 @@ def test_mfma_path_at_the_full_c2_size():
           flush=True)
     assert np.all(err <= tol)
     assert np.all(it % 20 == 0) and it.max() < 1000
-    for t in np.linspace(0, len(sup) - 1, 16).astype(int):
+    picks = np.linspace(0, len(sup) - 1, 16).astype(int)
+    for t in picks:
         e, c = plan_l1(C, a, sup[t], wts[t], f[t], gs[t], reg)
         assert e <= tol * 1.0001, (t, e)
         assert abs(err[t] - e) <= 1e-9 + 1e-6 * e, (t, err[t], e)
         assert abs(loss[t, 0] - c) <= 1e-9 * abs(c), (t, loss[t, 0], c)
         _, rl, mag = plan_loss(C, sup[t], f[t], gs[t], reg)
         assert abs(loss[t, 1] - rl) <= 1e-12 * mag, (t, loss[t, 1], rl)
+    # the same 16 targets in the FP64 batched form (the reference's FP64
+    # log-domain iteration), run to the same L1 tolerance: transport cost and
+    # regularised loss within 1e-5
+    tb16 = RB.TargetBatch([sup[t] for t in picks], [wts[t] for t in picks])
+    ref = host(RB.sinkhorn_batched(Cd, ad, tb16, reg, precision="fp64", err="l1",
+                                   zero_tol=tol, max_iter=1000), tb16)
+    for q, t in enumerate(picks):
+        assert ref[3][q] <= tol
+        for k in (0, 1):
+            d = abs(loss[t, k] - ref[4][q, k]) / abs(ref[4][q, k])
+            assert d <= 1e-5, (t, k, loss[t, k], ref[4][q, k], d)
+
+
+def test_fp64_batched_at_the_full_c2_size_against_oracle():
+    """Two targets of the full C2 workload (4096 x 205 reduced problems) in
+    the FP64 batched form against the oracle's log Sinkhorn
+    (src/sinkhorn_impl_log.cpp:225-376): same iteration count, f and g to
+    1e-10 relative."""
+    C, a, sup, wts, reg = workloads.c2()
+    picks = [0, 777]
+    tb = RB.TargetBatch([sup[t] for t in picks], [wts[t] for t in picks])
+    Cd, ad = to_dev(C, a)
+    f, gs, it, err, loss = host(RB.sinkhorn_batched(Cd, ad, tb, reg, precision="fp64",
+                                                    err="reference"), tb)
+    for q, t in enumerate(picks):
+        bfull = np.zeros(C.shape[0])
+        bfull[sup[t]] = wts[t]
+        o = O.sinkhorn_log(a, bfull, C, reg, n_threads=8)
+        print(f"target {t}: iterations {it[q]} (oracle {o['iter']})", flush=True)
+        assert it[q] == o["iter"]
+        assert np.abs(f[q] - o["f"]).max() / np.abs(o["f"]).max() < 1e-10
+        og = o["g"][sup[t]]
+        assert np.abs(gs[q] - og).max() / np.abs(og).max() < 1e-10
 
 
 def ref_err(C, a, sup, w, f, g, reg):

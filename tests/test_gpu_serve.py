@@ -331,3 +331,94 @@ def test_graph_replay_equals_eager(clouds):
         for key in ("f", "g", "b"):
             if key in e:
                 assert np.array_equal(e[key], g[key]), key
+
+
+# ------------------------------------------------------ C5 at its workload sizes
+
+
+def bary_host_err(C, eps, F, G, A, w):
+    """The engine's barycenter stopping error, recomputed on the host in FP64
+    from the returned potentials: sum_s |P_s 1 - A_s|_1 + |P_s^T 1 - b|_1 with
+    b = prod_s (P_s^T 1 .* exp(-G_s / eps))^{w_s} (the re-estimated common
+    target); also b normalised."""
+    S = A.shape[1]
+    err, cols = 0.0, []
+    for s in range(S):
+        P = np.exp((F[:, s, None] + G[None, :, s] - C) / eps)
+        err += np.abs(P.sum(1) - A[:, s]).sum()
+        cols.append(P.sum(0))
+    lb = sum(w[s] * (np.log(cols[s]) - G[:, s] / eps) for s in range(S))
+    b = np.exp(lb)
+    err += sum(np.abs(c - b).sum() for c in cols)
+    return err, b / b.sum()
+
+
+@pytest.fixture(scope="module")
+def c5_pick():
+    from rwig_amd import workloads
+    sizes = workloads.c5_sizes()
+    big = int(np.argmax(sizes))  # n = 7317, the largest of the 32 clouds
+    mid = next(k for k, n in enumerate(sizes) if 1000 <= n <= 2000)  # n = 1418
+    clouds = workloads.c5_clouds(only={big, mid})
+    return clouds, big, mid
+
+
+def test_c5_largest_clouds_host_recomputed(c5_pick):
+    clouds, big, mid = c5_pick
+    rng = np.random.default_rng(2008)
+    jobs = []
+    with ServeSession(hbm_budget_bytes=32 << 30) as s:
+        ids = {k: s.register_cost(clouds[k]) for k in (big, mid)}
+        nb, nm = clouds[big].shape[0], clouds[mid].shape[0]
+        for eps in (0.1, 0.01):
+            a, b = rng.dirichlet(np.ones(nb)), rng.dirichlet(np.ones(nb))
+            jobs.append(("s", big, eps, a, b, s.submit_sinkhorn(ids[big], a, b, eps)))
+            A = np.asfortranarray(rng.dirichlet(np.ones(nb), size=8).T)
+            w = rng.dirichlet(np.ones(8))
+            jobs.append(("b", big, eps, A, w, s.submit_barycenter(ids[big], A, w, eps)))
+        A = np.asfortranarray(rng.dirichlet(np.ones(nm), size=8).T)
+        w = rng.dirichlet(np.ones(8))
+        jobs.append(("b", mid, 0.01, A, w, s.submit_barycenter(ids[mid], A, w, 0.01)))
+        a, b = rng.dirichlet(np.ones(nm)), rng.dirichlet(np.ones(nm))
+        jobs.append(("s", mid, 0.01, a, b, s.submit_sinkhorn(ids[mid], a, b, 0.01)))
+        res = [s.result(j[-1]) for j in jobs]
+    for (kind, k, eps, x, y, _), r in zip(jobs, res):
+        C = clouds[k]
+        assert r["return_status"] == 0 and r["err"] <= 1e-6
+        if kind == "s":
+            e = marginal_err(C, eps, r["f"], r["g"], x, y)
+        else:
+            e, bn = bary_host_err(C, eps, r["F"], r["G"], x, y)
+            assert np.abs(bn - r["b"]).sum() <= 1e-12
+        print(f"C5 {kind} n={C.shape[0]} eps={eps}: iter {r['iter']} err {r['err']:.3e} "
+              f"host {e:.3e}", flush=True)
+        assert e <= 1e-6 * 1.0001 and abs(e - r["err"]) <= 1e-9
+    # the mid-size Sinkhorn against the oracle run to 1e-12
+    (_, k, eps, a, b, _), r = jobs[-1], res[-1]
+    o = O.sinkhorn_log(a, b, clouds[k], eps, maxiter=5000, zerotol=1e-12, n_threads=8)
+    assert abs(r["loss"] - o["loss"]) / abs(o["loss"]) < TOL_LOSS
+
+
+def test_c5_alone_equals_mix_at_the_largest_n(c5_pick):
+    clouds, big, mid = c5_pick
+    rng = np.random.default_rng(2009)
+    nb, nm = clouds[big].shape[0], clouds[mid].shape[0]
+    a, b = rng.dirichlet(np.ones(nb)), rng.dirichlet(np.ones(nb))
+    with ServeSession(hbm_budget_bytes=32 << 30) as s:
+        ids = {k: s.register_cost(clouds[k]) for k in (big, mid)}
+        alone = s.result(s.submit_sinkhorn(ids[big], a, b, 0.01))
+    with ServeSession(hbm_budget_bytes=32 << 30) as s:
+        ids = {k: s.register_cost(clouds[k]) for k in (big, mid)}
+        others = [s.submit_sinkhorn(ids[big], rng.dirichlet(np.ones(nb)),
+                                    rng.dirichlet(np.ones(nb)), 0.01),
+                  s.submit_barycenter(ids[big], np.asfortranarray(
+                      rng.dirichlet(np.ones(nb), size=4).T), rng.dirichlet(np.ones(4)), 0.01)]
+        probe = s.submit_sinkhorn(ids[big], a, b, 0.01)
+        others += [s.submit_sinkhorn(ids[mid], rng.dirichlet(np.ones(nm)),
+                                     rng.dirichlet(np.ones(nm)), eps) for eps in (0.1, 0.01)]
+        mix = s.result(probe)
+        for rid in others:
+            assert s.result(rid)["return_status"] == 0
+    assert mix["iter"] == alone["iter"] and mix["err"] == alone["err"]
+    assert np.array_equal(mix["f"], alone["f"]) and np.array_equal(mix["g"], alone["g"])
+    assert mix["loss"] == alone["loss"]

@@ -16,6 +16,12 @@ from oracle import oracle as O  # noqa: E402
 from rwig_amd._lib import check, lib  # noqa: E402
 
 
+# FP32 Gibbs products and recurrences at the WDL configuration (eps 0.1, C in
+# [0, 4]): the rounding of K = exp(-C/eps) in FP32 is ~6e-8 relative, and the
+# 2L - 1 products of the adjoint carry it
+TOL_C4_FP32 = 1e-4
+
+
 def wdl_problem(seed, N, M, S):
     rng = np.random.default_rng(seed)
     E = rng.standard_normal((N, 5))
@@ -111,3 +117,92 @@ def test_wdl_seeded_entry_point():
     np.testing.assert_allclose(r1["A"].sum(0), 1, rtol=1e-12)
     np.testing.assert_allclose(r1["W"].sum(0), 1, rtol=1e-12)
     np.testing.assert_allclose(r1["Yhat"].sum(0), 1, rtol=1e-12)
+
+
+# ------------------------------------------------------ C4 at its workload size
+
+
+class C4Batch:
+    """The WDL configuration's batch on the device: V = 2000 words, K = 16
+    atoms, D = 512 documents, eps 0.1, L = 20."""
+
+    def __init__(self):
+        from rwig_amd import workloads
+        C, Y, Alpha0, Lambda0, eps, L = workloads.c4()
+        self.N, self.D = Y.shape
+        self.S = Alpha0.shape[1]
+        self.L = L
+        self.Y, self.Alpha0, self.Lambda0 = Y, Alpha0, Lambda0
+        K = np.exp(-C / eps)
+        self.ldk = (self.N + 7) // 8 * 8
+        self.dev = {}
+        for prec, dt in ((0, torch.float64), (1, torch.float32)):
+            Kt = torch.zeros((self.N, self.ldk), dtype=dt)
+            Kt[:, :self.N] = torch.from_numpy(K.T.copy())
+            KTt = torch.zeros((self.N, self.ldk), dtype=dt)
+            KTt[:, :self.N] = torch.from_numpy(K.copy())
+            self.dev[prec] = (Kt.cuda(), KTt.cuda(),
+                              torch.from_numpy(Y.T.copy()).to(dt).cuda())
+
+    def grad(self, Alpha, Lambda, precision=0):
+        Kd, KTd, Yd = self.dev[precision]
+        dt = Yd.dtype
+        A = O.softmax_cols(Alpha)
+        W = O.softmax_cols(Lambda)
+        At = torch.from_numpy(A.T.copy()).to(dt).cuda()
+        Wt = torch.from_numpy(W.T.copy()).to(dt).cuda()
+        gA = torch.zeros((self.S, self.N), dtype=torch.float64, device="cuda")
+        gl = torch.zeros(self.S, dtype=torch.float64, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        check(lib().rwig_wdl_batch_grad_dev(Kd.data_ptr(), KTd.data_ptr(), self.ldk, self.N,
+                                            self.S, self.D, self.L, At.data_ptr(),
+                                            Yd.data_ptr(), Wt.data_ptr(), gA.data_ptr(),
+                                            gl.data_ptr(), loss.data_ptr(), precision,
+                                            torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        return gA.cpu().numpy().T, gl.cpu().numpy(), loss.item()
+
+
+@pytest.fixture(scope="module")
+def c4batch():
+    return C4Batch()
+
+
+def test_c4_fp64_gradient_matches_finite_differences(c4batch):
+    # the gradient is of the batch's mean loss (1/D) sum_d ||b_L,d - y_d||^2
+    # (src/wdl_impl.cpp:236-246); g_lambda along a shift of row s of every
+    # column of Lambda at once
+    b, h = c4batch, 1e-6
+    gA, gl, _ = b.grad(b.Alpha0, b.Lambda0)
+    mean = lambda Al, La: b.grad(Al, La)[2] / b.D  # noqa: E731
+    rng = np.random.default_rng(40)
+    coords = [(int(rng.integers(b.N)), s) for s in rng.choice(b.S, 8, replace=False)]
+    fd, an = [], []
+    for i, s in coords:
+        Ap, Am = b.Alpha0.copy(), b.Alpha0.copy()
+        Ap[i, s] += h
+        Am[i, s] -= h
+        fd.append((mean(Ap, b.Lambda0) - mean(Am, b.Lambda0)) / (2 * h))
+        an.append(gA[i, s])
+    for s in range(b.S):
+        Lp, Lm = b.Lambda0.copy(), b.Lambda0.copy()
+        Lp[s] += h
+        Lm[s] -= h
+        fd.append((mean(b.Alpha0, Lp) - mean(b.Alpha0, Lm)) / (2 * h))
+        an.append(gl[s])
+    fd, an = np.array(fd), np.array(an)
+    err = np.linalg.norm(fd - an) / np.linalg.norm(fd)
+    print(f"C4 FD check: |fd - adjoint| / |fd| = {err:.2e} over {len(fd)} coordinates "
+          f"(|g| {np.abs(fd).min():.1e} .. {np.abs(fd).max():.1e})", flush=True)
+    assert err < 1e-5
+
+
+def test_c4_fp32_against_fp64(c4batch):
+    b = c4batch
+    gA64, gl64, l64 = b.grad(b.Alpha0, b.Lambda0, 0)
+    gA32, gl32, l32 = b.grad(b.Alpha0, b.Lambda0, 1)
+    eA = np.linalg.norm(gA32 - gA64) / np.linalg.norm(gA64)
+    el = np.linalg.norm(gl32 - gl64) / np.linalg.norm(gl64)
+    eloss = abs(l32 - l64) / abs(l64)
+    print(f"C4 FP32 vs FP64: g_Alpha {eA:.2e}, g_lambda {el:.2e}, loss {eloss:.2e}", flush=True)
+    assert max(eA, el, eloss) < TOL_C4_FP32

@@ -55,6 +55,75 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+C5_ARRIVAL_RATE = 1200.0  # requests/s of the paced serving run, fixed across runs
+
+
+def host_info():
+    """Threads the CPU baselines use (the process's CPU affinity, at most 16:
+    the host share of one GPU), with nproc and the CPU model."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, min(16, aff)), "affinity_cpus": aff,
+            "nproc": os.cpu_count(), "cpu_model": model}
+
+
+def c1_leg(log, cpu=True, reps=50):
+    """Plain vanilla Sinkhorn (C1): n = m = 256 on the 1-D grid, eps 0.05, up
+    to 1000 iterations, stopping on the reference's error at 1e-9. The GPU
+    form is the drop-in rwig_sinkhorn_vanilla (host buffers in and out; the
+    whole loop is one persistent workgroup); the CPU form the oracle's
+    restatement of Sinkhorn::compute_vanilla on one thread (the reference
+    runs BLAS single-threaded)."""
+    import rwig_amd
+    from rwig_amd.workloads import c1
+    a, b, C, eps = c1()
+    ctrl = {"reg": eps, "method": "vanilla", "max_iter": 1000, "zero_tol": 1e-9}
+    out = {"workload": "vanilla Sinkhorn, n=m=256 on the 1-D grid, eps=0.05, <= 1000 "
+                       "iterations, reference error (L2 residuals) < 1e-9"}
+    for wg in (False, True):
+        ctrl["with_grad"] = wg
+        r = rwig_amd.sinkhorn(a, b, C, ctrl)
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = rwig_amd.sinkhorn(a, b, C, ctrl)
+        dt = (time.perf_counter() - t) / reps
+        key = "with_grad" if wg else "forward"
+        out[key] = {"ms_per_solve": dt * 1e3, "iterations": int(r["iter"]),
+                    "us_per_iteration": dt / max(1, r["iter"]) * 1e6,
+                    "solves_per_s": 1.0 / dt, "err": r["err"],
+                    "return_status": int(r["return_status"])}
+        P = r["P"]
+        out[key]["l1_marginal_err"] = float(np.abs(P.sum(1) - a).sum()
+                                            + np.abs(P.sum(0) - b).sum())
+        log(f"C1 GPU {key}: {dt * 1e3:.3f} ms per solve, {r['iter']} iterations, "
+            f"{dt / max(1, r['iter']) * 1e6:.2f} us per iteration")
+    if cpu:
+        from oracle import oracle as O
+        n = 20
+        t = time.perf_counter()
+        for _ in range(n):
+            o = O.sinkhorn_vanilla(a, b, C, eps, maxiter=1000, zerotol=1e-9)
+        dt = (time.perf_counter() - t) / n
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "solves/s", "cores": 1,
+                               "kind": "port", "ms_per_solve": dt * 1e3,
+                               "us_per_iteration": dt / max(1, o["iter"]) * 1e6,
+                               "sample": f"{n} solves of the C1 problem, C++ oracle "
+                                         "(Sinkhorn::compute_vanilla restated), one thread"}
+        log(f"C1 CPU oracle: {dt * 1e3:.3f} ms per solve, {o['iter']} iterations")
+    return out
+
+
 def c2_gemm(torch, dev, stream, product_only=True):
     """The dominant kernel of the MFMA path at its full shape: the BF16 product
     with M = K = 4096 (the cost's points) and N = 1024 (the targets), FP32
@@ -108,7 +177,7 @@ def gemm_only():
           "5 FP64 products 4096 x 1024 x 4096", flush=True)
 
 
-def c3_leg(log):
+def c3_leg(log, cpu=False):
     """Barycenter with unrolled gradient (64 x 64 grid, S = 8, eps 0.004,
     L = 200, checkpoints every 20): one forward + adjoint in FP32, in the
     mixed form (FP32 exponentials, FP64 arguments) and in FP64."""
@@ -133,6 +202,29 @@ def c3_leg(log):
         out[f"{prec}_ms_per_gradient"] = (time.perf_counter() - t) / reps * 1e3
         log(f"C3 {prec}: {out[f'{prec}_ms_per_gradient']:.1f} ms per forward + adjoint "
             "(host FP64 inputs and outputs included)")
+    if cpu:
+        from oracle import oracle as O
+        th = host_info()["threads"]
+
+        def run(nt, LL):
+            t1 = time.perf_counter()
+            O.barycenter_log(A, C, w, bext, eps, withgrad=True, n_threads=nt, maxiter=LL,
+                             zerotol=0.0)
+            return time.perf_counter() - t1
+        t1, t2 = run(th, 1), run(th, 2)
+        per_it = t2 - t1
+        est = (t1 - per_it) + L * per_it
+        t1s = run(0, 1)
+        est_s = t1s * est / t1
+        out["cpu_baseline"] = {
+            "value": 1.0 / est, "unit": "gradients/s", "cores": th, "kind": "port",
+            "s_per_gradient": est, "s_per_gradient_n_threads_0": est_s,
+            "sample": f"C++ oracle log barycenter with gradient at the full size, L = 1 and "
+                      f"L = 2 with n_threads={th} ({t1:.1f} s, {t2:.1f} s): the difference is "
+                      f"one iteration, extrapolated to L = {L}; n_threads=0 (serial) timed "
+                      f"at L = 1 ({t1s:.1f} s) and scaled by the same ratio"}
+        log(f"C3 CPU oracle: {est:.0f} s per gradient at {th} threads, {est_s:.0f} s serial "
+            "(extrapolated)")
     return out
 
 
@@ -182,7 +274,7 @@ def lse_leg(torch, dev, stream, log):
     return res
 
 
-def c4_leg(torch, dev, stream, log):
+def c4_leg(torch, dev, stream, log, cpu=False):
     """WDL training step (V = 2000, K = 16 atoms, 512 documents, eps 0.1, L = 20):
     one batched gradient over all documents (the step's dominant work; the Adam
     update is O(V K + K D)), FP32 and FP64, inputs resident in HBM."""
@@ -232,6 +324,20 @@ def c4_leg(torch, dev, stream, log):
         out[f"{prec}_ms_per_step"] = ms
         out[f"{prec}_docs_per_s"] = D / ms * 1e3
         log(f"C4 {prec}: {ms:.2f} ms per batched gradient ({D / ms * 1e3:.0f} documents/s)")
+    if cpu:
+        from oracle import oracle as O
+        d = 2
+        t1 = time.perf_counter()
+        O.wdl_batch_grad(K, A, Y[:, :d], Wb[:, :d], L)
+        dt = time.perf_counter() - t1
+        out["cpu_baseline"] = {
+            "value": d / dt, "unit": "documents/s", "cores": 1, "kind": "port",
+            "s_per_step_extrapolated": dt * D / d,
+            "sample": f"C++ oracle batched gradient (WDL::_train_batch_batched restated, one "
+                      f"thread as the reference's BLAS) on {d} of the {D} documents: "
+                      f"{dt:.1f} s, linear in the documents"}
+        log(f"C4 CPU oracle: {d / dt:.2f} documents/s ({dt * D / d:.0f} s per 512-document "
+            "step, extrapolated)")
     return out
 
 
@@ -288,7 +394,8 @@ def c5_leg(log, paced_s=8.0, graphs=True):
     """Serving mix: 32 cost matrices (n log-uniform in [64, 8192]) resident,
     70% Sinkhorn / 30% barycenter (S in {2, 4, 8}), eps in {0.1, 0.01}, stop at
     L1 marginal error 1e-6. Saturated throughput (1024 requests at once), then
-    Poisson arrivals at 70% of it for paced_s seconds: p50/p99 latency."""
+    Poisson arrivals at a fixed rate (C5_ARRIVAL_RATE) for paced_s seconds:
+    p50/p99 latency comparable across runs."""
     from rwig_amd import workloads as W
     from rwig_amd.serve import ServeSession
     clouds = W.c5_clouds()
@@ -315,13 +422,27 @@ def c5_leg(log, paced_s=8.0, graphs=True):
         rids = [submit(s, ids, q) for q in reqs]
         res = [s.result(r) for r in rids]
         sat = len(reqs) / (time.perf_counter() - t)
-        ok = all(r["err"] <= 1e-6 for r in res)
+        ok = all(r["err"] <= 1e-6 and r["return_status"] == 0 for r in res)
+        # the stopping error recomputed on the host in FP64 for a sample of
+        # Sinkhorn results (the returned potentials' plan)
+        rng_chk = np.random.default_rng(5)
+        small = [k for k, q in enumerate(reqs) if q["kind"] == "sinkhorn"
+                 and sizes[q["cloud"]] <= 2048]
+        worst = 0.0
+        for k in rng_chk.choice(small, min(16, len(small)), replace=False):
+            q, r = reqs[k], res[k]
+            Ck = clouds[q["cloud"]]
+            P = np.exp((r["f"][:, None] + r["g"][None, :] - Ck) / q["eps"])
+            e = np.abs(P.sum(1) - q["a"]).sum() + np.abs(P.sum(0) - q["b"]).sum()
+            worst = max(worst, e)
+        ok = bool(ok and worst <= 1e-6 * 1.0001)
         st = s.stats()
+        out.update(host_checked_sample=16, host_checked_max_err=float(worst))
         out.update(saturated_solves_per_s=sat, saturated_p50_ms=st["p50_ms"],
                    saturated_p99_ms=st["p99_ms"], all_converged_saturated=ok)
         log(f"C5 saturated: {sat:.0f} solves/s, p50 {st['p50_ms']:.0f} ms, "
             f"p99 {st['p99_ms']:.0f} ms, all converged {ok}")
-        rate = 0.7 * sat
+        rate = C5_ARRIVAL_RATE
         rng = np.random.default_rng(2006)
         reqs = list(W.c5_requests(sizes, int(rate * paced_s), seed=2007))
         s.reset_stats()
@@ -336,7 +457,7 @@ def c5_leg(log, paced_s=8.0, graphs=True):
             rids.append(submit(s, ids, q))
         res = [s.result(r) for r in rids]
         st = s.stats()
-        ok = all(r["err"] <= 1e-6 for r in res)
+        ok = all(r["err"] <= 1e-6 and r["return_status"] == 0 for r in res)
         out.update(arrival_rate_per_s=rate, requests=len(reqs),
                    paced_solves_per_s=st["solves_per_s"], p50_ms=st["p50_ms"],
                    p99_ms=st["p99_ms"], mean_queue_ms=st["mean_queue_ms"],
@@ -360,7 +481,7 @@ def main():
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the dominant product (5 launches) and a 256 MiB copy (5 "
                          "launches): the workload of the HBM-traffic counter runs")
-    ap.add_argument("--only", choices=("c3", "lse", "c4", "c5", "bary"),
+    ap.add_argument("--only", choices=("c1", "c3", "lse", "c4", "c5", "bary"),
                     help="run one secondary measurement alone and print its JSON object "
                          "(the workload of a profiler run)")
     ap.add_argument("--serve-eager", action="store_true",
@@ -378,7 +499,8 @@ def main():
         import torch
         dev = torch.device("cuda:0")
         stream = torch.cuda.current_stream()
-        legs = {"c3": lambda: c3_leg(log), "lse": lambda: lse_leg(torch, dev, stream, log),
+        legs = {"c1": lambda: c1_leg(log, cpu=False), "c3": lambda: c3_leg(log),
+                "lse": lambda: lse_leg(torch, dev, stream, log),
                 "c4": lambda: c4_leg(torch, dev, stream, log),
                 "c5": lambda: c5_leg(log, graphs=not args.serve_eager),
                 "bary": lambda: bary_leg(log)}
@@ -518,7 +640,8 @@ def main():
     cpu = None
     if not args.no_cpu and args.cpu_sample > 0:
         from oracle import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        hinfo = host_info()
+        cores = hinfo["threads"]
         nc = min(args.cpu_sample, B)
         log(f"CPU oracle (log Sinkhorn, {cores} threads) on {nc} targets")
         t1 = time.perf_counter()
@@ -527,15 +650,17 @@ def main():
             O.sinkhorn_log(a, wts[t], Ct, reg, n_threads=cores, maxiter=1000, zerotol=tol)
         dt = time.perf_counter() - t1
         cpu = {"value": nc / dt, "unit": "solves/s", "cores": cores, "kind": "port",
+               "host": hinfo,
                "sample": f"{nc} of the {B} targets, reference log-Sinkhorn algorithm "
                          f"(C++ oracle, n_threads={cores}, zero_tol=1e-6 on the "
                          "reference error), n=4096 x 205 reduced problems"}
 
     extra = {}
     if not args.no_extra:
-        extra["c3_barycenter_gradient"] = c3_leg(log)
+        extra["c1_vanilla"] = c1_leg(log, cpu=not args.no_cpu)
+        extra["c3_barycenter_gradient"] = c3_leg(log, cpu=not args.no_cpu)
         extra["lse_roofline"] = lse_leg(torch, dev, stream, log)
-        extra["c4_wdl_step"] = c4_leg(torch, dev, stream, log)
+        extra["c4_wdl_step"] = c4_leg(torch, dev, stream, log, cpu=not args.no_cpu)
         extra["c5_serving"] = c5_leg(log)
         extra["barycenter_batched"] = bary_leg(log)
 

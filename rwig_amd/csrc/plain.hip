@@ -145,6 +145,176 @@ struct SinkhornResult {
   double err = 0.0;
 };
 
+// ---------------------------------------------------------------- small problems
+//
+// A problem whose Gibbs kernel fits one XCD's L2 with room to spare runs its
+// whole forward loop in ONE workgroup of 1024 threads: K and K^T, built
+// beforehand over the whole GPU, are read from that CU's L2 after the first
+// iteration, the vectors u, v, K v, K^T u live in LDS, and the loop condition of the reference,
+// while (iter < maxiter && err >= tol), is decided by thread 0 and read by
+// every thread after a workgroup barrier, so the loop ends for all waves
+// together and within maxiter iterations. The multi-kernel form pays five
+// launches and a host poll per batch of iterations instead; at 256 x 256
+// those cost ~24 us per iteration. Reductions run in a fixed order.
+constexpr int kTinyTB = 1024;
+constexpr int kTinyWaves = kTinyTB / kWave;
+constexpr long kTinyMaxMN = 1L << 17;  // 1 MiB of FP64 K
+constexpr int kTinyMaxMplusN = 2048;   // 4 vectors and the pass partials in LDS: < 48 KiB
+
+inline bool tiny_vanilla(int M, int N) {
+  return (long)M * N <= kTinyMaxMN && M + N <= kTinyMaxMplusN;
+}
+
+// Workgroup sum in a fixed order; every thread gets the result.
+__device__ __forceinline__ double tiny_sum(double x, double *red) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  x = wave_sum(x);
+  __syncthreads();
+  if (lane == 0) red[w] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kTinyWaves; ++k) t += red[k];
+  return t;
+}
+
+// out[r] = sum_c A[r + c * R] x[c] for r < R (A column-major R x Cn, in
+// global memory and L2-resident), by the whole workgroup: G = kTinyTB / R
+// column groups when R <= kTinyTB (their partials summed in group order),
+// else rows looped. Each thread keeps four accumulators and unrolls its loads
+// so that many are in flight: the pass is L2-latency-bound, not
+// bandwidth-bound, at these sizes.
+__device__ __forceinline__ void tiny_row_pass(const double *__restrict__ A, int R, int Cn,
+                                              const double *x, double *out, double *part) {
+  const int tid = threadIdx.x;
+  auto dot = [&](int r, int c0, int c1) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int c = c0;
+    for (; c + 7 < c1; c += 8) {
+      double k[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) k[q] = A[r + (long)(c + q) * R];  // 8 loads in flight
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
+    }
+    for (; c < c1; ++c) s[0] = fma(A[r + (long)c * R], x[c], s[0]);
+    return (s[0] + s[1]) + (s[2] + s[3]);
+  };
+  __syncthreads();  // A and x complete
+  if (R <= kTinyTB) {
+    const int G = kTinyTB / R, cw = (Cn + G - 1) / G;
+    const int r = tid % R, g = tid / R;
+    if (g < G) part[g * R + r] = dot(r, min(Cn, g * cw), min(Cn, (g + 1) * cw));
+    __syncthreads();
+    if (tid < R) {
+      double acc = 0.0;
+      for (int q = 0; q < G; ++q) acc += part[q * R + tid];
+      out[tid] = acc;
+    }
+  } else {
+    for (int r = tid; r < R; r += kTinyTB) out[r] = dot(r, 0, Cn);
+  }
+  __syncthreads();
+}
+
+// K = exp(-C / reg) and K^T, 32 x 32 tiles through LDS (reads of C and
+// writes of both coalesced), over the whole GPU: the one-workgroup loop then
+// reads both from L2.
+__global__ void __launch_bounds__(1024)
+k_gibbs_both(const double *__restrict__ C, int M, int N, double reg, double *__restrict__ K,
+             double *__restrict__ KT) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int i = i0 + tx, j = j0 + ty;
+  double kv = 0.0;
+  if (i < M && j < N) {
+    kv = exp(-C[i + (long)j * M] / reg);
+    K[i + (long)j * M] = kv;
+  }
+  tile[ty][tx] = kv;
+  __syncthreads();
+  const int jt = j0 + tx, it = i0 + ty;
+  if (jt < N && it < M) KT[jt + (long)it * N] = tile[tx][ty];
+}
+
+// Forward of Sinkhorn::_fwd_vanilla (src/sinkhorn_impl_vanilla.cpp:93-166):
+// K = exp(-C/reg); Kv = K 1; loop { u = a / Kv; z = K^T u; v = b / z;
+// Kv = K v; err = ||u .* Kv - a||_2 + ||v .* z - b||_2 } over K and its
+// transpose KT (k_gibbs_both; both read from L2 by the row passes). Writes
+// u, v and the final (iter, err) to ls; with uh non-null also the history slices u_l, v_l,
+// K v_l (yh) and K^T u_l (zh) the adjoint reads.
+__global__ void __launch_bounds__(kTinyTB)
+k_sinkhorn_vanilla_tiny(const double *__restrict__ a, const double *__restrict__ b,
+                        const double *__restrict__ C, int M, int N, double reg, int maxiter,
+                        double tol, const double *__restrict__ K,
+                        const double *__restrict__ KT,
+                        double *__restrict__ u, double *__restrict__ v, LoopState *ls,
+                        double *uh, double *vh, double *yh, double *zh) {
+  extern __shared__ double lds[];
+  double *su = lds, *sKv = su + M, *sv = sKv + M, *sz = sv + N;
+  double *part = sz + N;  // max(kTinyTB, M, N) doubles: the column groups of a pass
+  __shared__ double red[kTinyWaves];
+  __shared__ int active;
+  const int tid = threadIdx.x;
+  const long MN = (long)M * N;
+  for (int j = tid; j < N; j += kTinyTB) sv[j] = 1.0;
+  tiny_row_pass(K, M, N, sv, sKv, part);
+  if (uh) {
+    for (int i = tid; i < M; i += kTinyTB) {
+      uh[i] = 1.0;
+      yh[i] = sKv[i];
+    }
+    for (int j = tid; j < N; j += kTinyTB) vh[j] = 1.0;
+  }
+  int iter = 0;
+  double err = 1000.0;  // inst/include/sinkhorn_impl.hpp:85-88
+  if (tid == 0) active = (iter < maxiter && err >= tol) ? 1 : 0;
+  __syncthreads();
+  while (active) {
+    const long l = iter + 1;
+    for (int i = tid; i < M; i += kTinyTB) {
+      const double ui = a[i] / sKv[i];
+      su[i] = ui;
+      if (uh) uh[l * M + i] = ui;
+    }
+    tiny_row_pass(KT, N, M, su, sz, part);  // z = K^T u
+    for (int j = tid; j < N; j += kTinyTB) {
+      const double vj = b[j] / sz[j];
+      sv[j] = vj;
+      if (uh) {
+        zh[l * N + j] = sz[j];
+        vh[l * N + j] = vj;
+      }
+    }
+    tiny_row_pass(K, M, N, sv, sKv, part);  // Kv = K v
+    double e1 = 0.0, e2 = 0.0;
+    for (int i = tid; i < M; i += kTinyTB) {
+      const double r = su[i] * sKv[i] - a[i];
+      e1 += r * r;
+      if (uh) yh[l * M + i] = sKv[i];
+    }
+    for (int j = tid; j < N; j += kTinyTB) {
+      const double r = sv[j] * sz[j] - b[j];
+      e2 += r * r;
+    }
+    e1 = tiny_sum(e1, red);
+    e2 = tiny_sum(e2, red);
+    ++iter;
+    err = sqrt(e1) + sqrt(e2);
+    if (tid == 0) active = (iter < maxiter && err >= tol) ? 1 : 0;
+    __syncthreads();
+  }
+  for (int i = tid; i < M; i += kTinyTB) u[i] = iter > 0 ? su[i] : 1.0;
+  for (int j = tid; j < N; j += kTinyTB) v[j] = sv[j];
+  if (tid == 0) {
+    ls->iter = iter;
+    ls->err = err;
+    ls->active = 0;
+    ls->maxiter = maxiter;
+    ls->tol = tol;
+  }
+}
+
 // All pointers are device pointers of the reduced (support) problem.
 // u (M), v (N), P (M*N), loss (1), grad (M, withgrad only) are outputs.
 SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const double *b,
@@ -167,6 +337,20 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
   }
   double *uh = uhb.get(), *vh = vhb.get(), *yh = yhb.get(), *zh = zhb.get();
 
+  LoopState fin{};
+  if (tiny_vanilla(M, N)) {
+    DBuf<double> KTb(MN);
+    const size_t lds = sizeof(double) * (2 * (size_t)M + 2 * (size_t)N +
+                                         std::max({kTinyTB, M, N}));
+    hipLaunchKernelGGL(k_gibbs_both, dim3((M + 31) / 32, (N + 31) / 32), dim3(1024), 0, st, C,
+                       M, N, reg, K, KTb.get());
+    check_launch();
+    hipLaunchKernelGGL(k_sinkhorn_vanilla_tiny, dim3(1), dim3(kTinyTB), lds, st, a, b, C, M, N,
+                       reg, maxiter, tol, K, KTb.get(), u, v, ls, uh, vh, yh, zh);
+    check_launch();
+    d2h(&fin, ls, 1, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  } else {
   launch_map(st, MN, [=] __device__(long k) { K[k] = exp(-C[k] / reg); });
   fill(st, u, M, 1.0);
   fill(st, v, N, 1.0);
@@ -230,7 +414,8 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
         },
         [=] __device__(double s1, double s2) { finish_iteration(ls, sqrt(s1) + sqrt(s2)); }, ls);
   };
-  const LoopState fin = run_loop(st, ls, maxiter, iteration);
+  fin = run_loop(st, ls, maxiter, iteration);
+  }
 
   // P = diag(u) K diag(v)
   launch_map(st, MN, [=] __device__(long k) {

@@ -90,6 +90,10 @@ def test_golden_barycenter(golden):
     (2, 300, 257, 10, 0.05, True),
     (3, 1, 9, 0, 0.2, False),
     (4, 700, 90, 0, 0.02, False),
+    # M > 1024 rows in the one-workgroup form (rows looped, no column groups)
+    (5, 1500, 80, 0, 0.05, True),
+    # past the one-workgroup limit (M N > 2^17): the multi-kernel form
+    (6, 600, 450, 5, 0.05, True),
 ])
 def test_sinkhorn_vanilla_matches_oracle(seed, M, N, zeros, reg, wg):
     a, b, C = problem(seed, M, N, zeros)
@@ -124,6 +128,13 @@ def test_sinkhorn_maxiter_exhaustion_status():
     a, b, C = problem(20, 30, 20)
     r = rwig_amd.sinkhorn(a, b, C, {"reg": 0.01, "method": "vanilla", "max_iter": 3})
     assert r["iter"] == 3 and r["return_status"] == 1
+    for M, N in [(30, 20), (600, 450)]:  # one-workgroup and multi-kernel forms
+        a, b, C = problem(21, M, N)
+        r = rwig_amd.sinkhorn(a, b, C, {"reg": 0.05, "method": "vanilla", "max_iter": 0})
+        o = O.sinkhorn_vanilla(a, b, C, 0.05, maxiter=0)
+        assert r["iter"] == o["iter"] == 0
+        for k in ["P", "u", "v", "loss"]:
+            assert rel(r[k], o[k]) < TOL_FP64, k
 
 
 def bary_problem(seed, M, N, S):

@@ -110,6 +110,20 @@ int rwig_barycenter_log_ck(double *F, double *G, double *b, double *grad_A,
                            double zero_tol, int checkpoint_every,
                            int seed_mode, int precision, int *return_status);
 
+/* D log barycenters without gradient that share A (M x S) and C (M x N),
+ * document d weighted by W(:, d) (S x D): B (N x D) receives each document's
+ * normalised barycenter as barycenter_log_cpp computes it
+ * (src/barycenter_impl_log.cpp:23-48, 189-272; spectral-norm stop per
+ * document), iters and errs (D each, may be null) their iteration counts and
+ * errors. The WDL inference of sinkhorn_mode 2 (src/wdl_impl.cpp:354-377),
+ * which the reference runs as one barycenter per document: here C stays on
+ * the device and the soft-min passes of 32 / S documents share one sweep of
+ * it. FP64, host buffers. */
+int rwig_barycenter_log_batched(double *B, int *iters, double *errs,
+                                const double *A, const double *W,
+                                const double *C, int M, int N, int S, int D,
+                                double reg, int max_iter, double zero_tol);
+
 /* rwig_sinkhorn_log on DEVICE pointers of an already reduced problem (every
  * a_i and b_j > 0): the per-call plain FP64 form with inputs resident in HBM.
  * P (m x n), f (m), g (n), loss (1 scalar) and grad_a (m, withgrad) are

@@ -33,6 +33,8 @@ SIGNATURES = {
                              _i, _i, _d, _i, _i, _d, _ip], _i),
     "rwig_barycenter_log_ck": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
                                 _i, _i, _i, _d, _i, _i, _d, _i, _i, _i, _ip], _i),
+    "rwig_barycenter_log_batched": ([_dp, _ip, _dp, _dp, _dp, _dp, _i, _i, _i, _i, _d, _i,
+                                     _d], _i),
     "rwig_sinkhorn_batched_dev": ([_vp, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp, _i, _d, _i,
                                    _d, _i, _i, _vp, ctypes.c_long, _i, _vp, _vp, _vp, _vp, _vp,
                                    _vp], _i),

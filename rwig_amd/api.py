@@ -263,8 +263,9 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
     precision "fp32" runs the Gibbs products and recurrences in FP32.
     sinkhorn_mode selects the inference of Yhat only, as in the reference
     (src/wdl_impl.cpp:354-377): 1 batches the parallel barycenters of all
-    documents on the GPU; 2 runs one log-domain barycenter (FP64, GPU) per
-    document with the reference's spectral-norm stopping rule."""
+    documents on the GPU; 2 batches their log-domain barycenters (FP64, GPU,
+    rwig_barycenter_log_batched) with the reference's spectral-norm stopping
+    rule per document."""
     if sinkhorn_mode not in (1, 2):
         raise ValueError("Sinkhorn mode not supported")
     if isinstance(optimizer, str):
@@ -298,7 +299,8 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
         check(lib().rwig_wdl_init(_p(A), _p(W), Yp, _p(Y), _p(C), N, M, int(S), *args,
                                   _p(Alpha0), _p(Lambda0), prec))
     if sinkhorn_mode == 2:
-        ctrl = {"reg": reg, "method": "log", "max_iter": max_iter, "zero_tol": zero_tol}
-        for m in range(M):
-            Yhat[:, m] = barycenter(A, C, W[:, m], barycenter_control=ctrl)["b"]
+        # every document's log barycenter in one call, C resident on the device
+        check(lib().rwig_barycenter_log_batched(_p(Yhat), None, None, _p(A), _p(W), _p(C), N, N,
+                                                int(S), M, float(reg), int(max_iter),
+                                                float(zero_tol)))
     return {"A": A, "W": W, "Yhat": Yhat}

@@ -343,6 +343,139 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
   return {L, hs.err};
 }
 
+// WDL inference in the log domain (src/wdl_impl.cpp:354-377): D barycenters
+// sharing A (M x S) and the cost, document d weighted by W(:, d), each the
+// log barycenter of src/barycenter_impl_log.cpp:189-272 without gradient,
+// stopping on its own spectral-norm error. Documents run in groups of
+// G = 32 / S (at least 1): the soft-min passes of a group sweep C once for
+// all its G S slices. A document that has stopped keeps its state while the
+// others go on (its updates are masked; a slice's arithmetic does not depend
+// on the other slices of a pass), so its result is the one it would have
+// alone. Device pointers: A (M x S), W (S x D) FP64; C (M x N) and CT (N x M)
+// of type T; B (N x D, normalised barycenters), iters, errs (D) out.
+template <typename T, typename X>
+void barycenter_log_batch(hipStream_t st, const double *A, const T *C, const T *CT,
+                          const double *W, int M, int N, int S, int D, double reg, int maxiter,
+                          double tol, double *B, int *iters, double *errs) {
+  const int G = std::max(1, 32 / S), QM = G * S;
+  const T treg = (T)reg;
+  Arena ar;
+  T *F = ar.make<T>((size_t)M * QM), *Gp = ar.make<T>((size_t)N * QM);
+  T *KV = ar.make<T>((size_t)M * QM), *logb = ar.make<T>((size_t)N * G);
+  T *wq = ar.make<T>(QM), *logA = ar.make<T>((size_t)M * S);
+  T *pm = ar.make<T>(lse_row_part_size(M, N, QM)), *ps = ar.make<T>(lse_row_part_size(M, N, QM));
+  T *cpm = ar.make<T>(col_t_part_size(M, N, QM)), *cps = ar.make<T>(col_t_part_size(M, N, QM));
+  const size_t gps = gram_part_size(S);
+  double *gpart = ar.make<double>(gps * G), *tot = ar.make<double>(G);
+  LoopState *ls = ar.make<LoopState>(G + 1);  // G documents, then the group's gate
+  LoopState *gate = ls + G;
+  launch_map(st, (long)M * S, [=] __device__(long k) { logA[k] = (T)log(A[k]); });
+  for (int d0 = 0; d0 < D; d0 += G) {
+    const int g = std::min(G, D - d0), Q = g * S;
+    launch_map(st, (long)M * Q, [=] __device__(long k) { F[k] = T(0); });
+    launch_map(st, (long)N * Q, [=] __device__(long k) { Gp[k] = T(0); });
+    launch_map(st, (long)N * g, [=] __device__(long k) { logb[k] = T(0); });
+    launch_map(st, (long)Q, [=] __device__(long q) {
+      wq[q] = (T)W[(q % S) + (long)(d0 + q / S) * S];
+    });
+    {
+      std::vector<LoopState> h(g + 1);
+      for (auto &x : h) {
+        x.iter = 0;
+        x.err = 1000.0;  // inst/include/sinkhorn_impl.hpp:85-88
+        x.maxiter = maxiter;
+        x.tol = tol;
+        x.active = (0 < maxiter && x.err >= tol) ? 1 : 0;
+      }
+      h2d(ls, h.data(), (size_t)g, st);
+      h2d(gate, &h[g], 1, st);
+    }
+    lse_row<T, X>(st, C, M, N, M, F, M, Gp, N, Q, reg, pm, ps,
+                  [=] __device__(long k, T r) { KV[k] = r; }, nullptr);
+    auto iteration = [&]() {
+      // F += reg log A + KV, for the slices of running documents
+      launch_map(
+          st, (long)M * Q,
+          [=] __device__(long k) {
+            const long q = k / M;
+            if (!ls[q / S].active) return;
+            F[k] = F[k] + (treg * logA[k % M + (q % S) * (long)M] + KV[k]);
+          },
+          gate);
+      // KTU = column soft-min; logb = -(G + KTU) w / reg; G += reg logb + KTU
+      lse_col_t<T, X>(
+          st, CT, M, N, N, F, M, Gp, N, Q, reg, cpm, cps,
+          [=] __device__(int j, const T *r) {
+            for (int d = 0; d < g; ++d) {
+              if (!ls[d].active) continue;
+              T acc = T(0);
+              for (int s = 0; s < S; ++s)
+                acc += -(Gp[j + (long)(d * S + s) * N] + r[d * S + s]) * wq[d * S + s];
+              const T lb = acc / treg;
+              logb[j + (long)d * N] = lb;
+              for (int s = 0; s < S; ++s) {
+                const long k = j + (long)(d * S + s) * N;
+                Gp[k] = Gp[k] + (treg * lb + r[d * S + s]);
+              }
+            }
+          },
+          gate);
+      // KV = row soft-min
+      lse_row<T, X>(
+          st, C, M, N, M, F, M, Gp, N, Q, reg, pm, ps,
+          [=] __device__(long k, T r) {
+            if (ls[(k / M) / S].active) KV[k] = r;
+          },
+          gate);
+      // each document's error: the spectral norm of -KV / reg - log A
+      for (int d = 0; d < g; ++d)
+        spectral(
+            st, (long)M, S,
+            [=] __device__(long i, int s) {
+              return -(double)KV[i + (long)(d * S + s) * M] / reg - (double)logA[i + (long)s * M];
+            },
+            [=] __device__(double nrm) { finish_iteration_ck(ls + d, nrm); }, ls + d,
+            gpart + d * gps);
+      // the group runs while one of its documents does
+      launch_map(
+          st, 1L,
+          [=] __device__(long) {
+            int a = 0;
+            for (int d = 0; d < g; ++d) a |= ls[d].active;
+            gate->iter = gate->iter + 1;
+            gate->active = a;
+          },
+          gate);
+    };
+    LoopState hs{};
+    d2h(&hs, gate, 1, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    for (int batch = 4; hs.active; batch = std::min(batch * 2, 64)) {
+      const int n = std::min(batch, maxiter - hs.iter);
+      for (int k = 0; k < n; ++k) iteration();
+      d2h(&hs, gate, 1, st);
+      RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    // b = exp(logb) / sum  (src/barycenter_impl_log.cpp:268-271)
+    for (int d = 0; d < g; ++d) {
+      const T *lb = logb + (long)d * N;
+      double *t = tot + d;
+      reduce2(
+          st, (long)N, [=] __device__(long j) { return exp((double)lb[j]); }, 0L,
+          [=] __device__(long) { return 0.0; }, [=] __device__(double s1, double) { *t = s1; });
+    }
+    launch_map(st, (long)N * g, [=] __device__(long k) {
+      const long d = k / N;
+      B[k + (long)d0 * N] = exp((double)logb[k]) / tot[d];
+    });
+    launch_map(st, (long)g, [=] __device__(long d) {
+      if (iters) iters[d0 + d] = ls[d].iter;
+      if (errs) errs[d0 + d] = ls[d].err;
+    });
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 template <typename T, typename X>
 BaryCkResult run_ck(hipStream_t st, double *Fh, double *Gh, double *bh, double *gAh, double *gwh,
                     double *lossh, const double *A, const double *w, const double *C,
@@ -390,6 +523,39 @@ BaryCkResult run_ck(hipStream_t st, double *Fh, double *Gh, double *bh, double *
 }  // namespace rwig
 
 using namespace rwig;
+
+extern "C" int rwig_barycenter_log_batched(double *B, int *iters_out, double *errs_out,
+                                           const double *A, const double *W, const double *C,
+                                           int M, int N, int S, int D, double reg,
+                                           int max_iter, double zero_tol) {
+  return guarded("rwig_barycenter_log_batched", [&] {
+    RWIG_REQUIRE(M > 0 && N > 0 && D >= 0, "matrix dimensions must be positive");
+    RWIG_REQUIRE(S > 0 && S <= 32, "S must be in [1, 32]");
+    RWIG_REQUIRE(max_iter >= 0, "max_iter must be non-negative");
+    RWIG_REQUIRE(reg > 0.0, "reg must be positive");
+    RWIG_REQUIRE(B && A && W && C, "B, A, W and C are required");
+    if (D == 0) return;
+    ensure_device();
+    Stream s;
+    hipStream_t st = s.get();
+    Arena ar;
+    const long MN = (long)M * N;
+    double *dA = ar.make<double>((size_t)M * S), *dW = ar.make<double>((size_t)S * D);
+    double *dC = ar.make<double>(MN), *dCT = ar.make<double>(MN);
+    double *dB = ar.make<double>((size_t)N * D), *dE = ar.make<double>(D);
+    int *dI = ar.make<int>(D);
+    h2d(dA, A, (size_t)M * S, st);
+    h2d(dW, W, (size_t)S * D, st);
+    h2d(dC, C, (size_t)MN, st);
+    transpose<double>(st, dC, M, N, M, dCT, N);
+    barycenter_log_batch<double, double>(st, dA, dC, dCT, dW, M, N, S, D, reg, max_iter,
+                                         zero_tol, dB, dI, dE);
+    d2h(B, dB, (size_t)N * D, st);
+    if (iters_out) d2h(iters_out, dI, (size_t)D, st);
+    if (errs_out) d2h(errs_out, dE, (size_t)D, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  });
+}
 
 extern "C" int rwig_barycenter_log_ck(double *F, double *G, double *b, double *grad_A,
                                       double *grad_w, double *loss, int *iter_out,

@@ -106,7 +106,8 @@ def test_wig_scores_match_oracle(svd_method):
 
 
 def test_wdl_log_mode_inference_matches_oracle():
-    # sinkhorn_mode 2 infers Yhat with one log-domain barycenter per document
+    # sinkhorn_mode 2 infers Yhat with every document's log-domain barycenter
+    # (rwig_barycenter_log_batched)
     rng = np.random.default_rng(8)
     N, S, M = 40, 3, 10
     x = np.sort(rng.random(N))
@@ -119,6 +120,34 @@ def test_wdl_log_mode_inference_matches_oracle():
     for m in range(M):
         ref = O.barycenter_log(fit["A"], C, fit["W"][:, m], reg=0.05, maxiter=20)["b"]
         assert np.abs(fit["Yhat"][:, m] - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("S,D", [(4, 9), (5, 7), (32, 2)])
+def test_batched_log_barycenters_match_oracle_per_document(S, D):
+    # rwig_barycenter_log_batched packs 32 // S documents per soft-min sweep;
+    # each must stop at its own iteration (spectral-norm error, tol 1e-6) with
+    # the oracle's barycenter_log result
+    import ctypes
+    from rwig_amd._lib import check, lib
+    rng = np.random.default_rng(100 + S)
+    M, N = 50, 45
+    x, y = rng.random((M, 2)), rng.random((N, 2))
+    C = np.asfortranarray(((x[:, None] - y[None]) ** 2).sum(-1))
+    A = np.asfortranarray(rng.dirichlet(np.ones(M), size=S).T)
+    W = np.asfortranarray(rng.dirichlet(np.ones(S), size=D).T)
+    B = np.zeros((N, D), order="F")
+    iters = np.zeros(D, dtype=np.int32)
+    errs = np.zeros(D)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    check(lib().rwig_barycenter_log_batched(
+        dp(B), iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), dp(errs), dp(A), dp(W), dp(C),
+        M, N, S, D, 0.05, 1000, 1e-6))
+    for d in range(D):
+        o = O.barycenter_log(A, C, W[:, d], reg=0.05, maxiter=1000, zerotol=1e-6)
+        assert iters[d] == o["iter"], (d, iters[d], o["iter"])
+        assert abs(errs[d] - o["err"]) <= 1e-6 * o["err"] + 1e-14  # a near-cancelling norm
+        assert np.abs(B[:, d] - o["b"]).max() <= 1e-10 * np.abs(o["b"]).max()
+    print(f"S={S} D={D}: iterations {iters.tolist()}")
 
 
 def test_wig_composition():

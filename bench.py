@@ -277,7 +277,8 @@ def lse_leg(torch, dev, stream, log):
 def c4_leg(torch, dev, stream, log, cpu=False):
     """WDL training step (V = 2000, K = 16 atoms, 512 documents, eps 0.1, L = 20):
     one batched gradient over all documents (the step's dominant work; the Adam
-    update is O(V K + K D)), FP32 and FP64, inputs resident in HBM."""
+    update is O(V K + K D)), FP32 and FP64, inputs resident in HBM; each also
+    with the adjoint's history checkpointed every 5 iterations."""
     from rwig_amd._lib import check, lib
     from rwig_amd.workloads import c4
     C, Y, Alpha0, Lambda0, eps, L = c4()
@@ -294,7 +295,16 @@ def c4_leg(torch, dev, stream, log, cpu=False):
     ldk = (V + 7) // 8 * 8
     out = {"workload": f"WDL batched gradient, V={V}, K={S} atoms, {D} documents, eps={eps}, "
                        f"L={L}"}
-    for prec, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+    def history_bytes(ck, size):
+        # adjoint history of the step: UB, VB, KTUB (SL + 1 slices), KVB (SL),
+        # bB, and the VB checkpoints when ck > 0
+        ldn = (V + 7) // 8 * 8
+        sl = ck if 0 < ck < L else L
+        slices = 3 * (sl + 1) + sl + (((L - 1) // ck + 1) if 0 < ck < L else 0)
+        return (slices * ldn * S * D + (sl + 1) * V * D) * size
+
+    for prec, dt, ck in (("fp32", torch.float32, 0), ("fp32", torch.float32, 5),
+                         ("fp64", torch.float64, 0), ("fp64", torch.float64, 5)):
         Kt = torch.zeros((V, ldk), dtype=dt)
         Kt[:, :V] = torch.from_numpy(K.T.copy())
         KTt = torch.zeros((V, ldk), dtype=dt)
@@ -312,7 +322,7 @@ def c4_leg(torch, dev, stream, log, cpu=False):
             check(lib().rwig_wdl_batch_grad_dev(Kd.data_ptr(), KTd.data_ptr(), ldk, V, S, D, L,
                                                 At.data_ptr(), Yt.data_ptr(), Wt.data_ptr(),
                                                 gA.data_ptr(), gl.data_ptr(), loss.data_ptr(), p,
-                                                stream.cuda_stream))
+                                                ck, stream.cuda_stream))
         step()
         torch.cuda.synchronize()
         reps = 5
@@ -321,9 +331,13 @@ def c4_leg(torch, dev, stream, log, cpu=False):
             step()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / reps * 1e3
-        out[f"{prec}_ms_per_step"] = ms
-        out[f"{prec}_docs_per_s"] = D / ms * 1e3
-        log(f"C4 {prec}: {ms:.2f} ms per batched gradient ({D / ms * 1e3:.0f} documents/s)")
+        key = prec if ck == 0 else f"{prec}_checkpoint{ck}"
+        hb = history_bytes(ck, 8 if prec == "fp64" else 4)
+        out[f"{key}_ms_per_step"] = ms
+        out[f"{key}_docs_per_s"] = D / ms * 1e3
+        out[f"{key}_history_bytes"] = hb
+        log(f"C4 {key}: {ms:.2f} ms per batched gradient ({D / ms * 1e3:.0f} documents/s), "
+            f"adjoint history {hb / 2**30:.2f} GiB")
     if cpu:
         from oracle import oracle as O
         d = 2

@@ -152,24 +152,30 @@ int rwig_wdl(double *A, double *W, double *Yhat, const double *Y,
 /* rwig_wdl from explicit initial Alpha0 (N x S) and Lambda0 (S x M) -- the
  * values the reference's CPU path draws with R's RNG
  * (src/wdl_impl.cpp:277-278). precision 0: FP64, 1: FP32 Gibbs products and
- * recurrences (optimizer, softmax in FP64). */
+ * recurrences (optimizer, softmax in FP64). checkpoint_every: 0 keeps the
+ * iterates of all max_iter steps for the adjoint, as the reference does
+ * (src/wdl_impl.cpp:111-121); k > 0 keeps VB every k steps and recomputes
+ * each segment in the adjoint -- the same results bit for bit, with
+ * k + max_iter/k + 1 slices of history instead of max_iter + 1 (each slice
+ * 4 N x S x B values), for max_iter more forward steps per batch. */
 int rwig_wdl_init(double *A, double *W, double *Yhat, const double *Y,
                   const double *C, int N, int M, int S, double reg,
                   int max_iter, double zero_tol, int B, int E, int opt,
                   double eta, double gamma, double beta1, double beta2,
                   double eps, const double *Alpha0, const double *Lambda0,
-                  int precision);
+                  int precision, int checkpoint_every);
 
 /* One batched WDL gradient on device pointers (src/wdl_impl.cpp:87-247):
  * K = exp(-C/reg) and KT its transpose, column-major, leading dimension ldk
  * (>= N, multiple of 8); A (N x S), Yb (N x D), Wb (S x D) in the precision
  * given (0 double, 1 float); outputs FP64: gAlpha (N x S), glambda (S) and
- * loss = sum ||b_L - y||^2 over the batch (may be null). */
+ * loss = sum ||b_L - y||^2 over the batch (may be null); checkpoint_every as
+ * in rwig_wdl_init. */
 int rwig_wdl_batch_grad_dev(const void *K, const void *KT, long ldk, int N,
                             int S, int D, int L, const void *A,
                             const void *Yb, const void *Wb, double *gAlpha,
                             double *glambda, double *loss, int precision,
-                            void *stream);
+                            int checkpoint_every, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Batched solves, device pointers (no counterpart in the reference, whose GPU

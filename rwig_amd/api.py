@@ -253,14 +253,17 @@ _OPTIMIZERS = {"sgd": 0, "adam": 1, "adamw": 2}
 
 def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, usecuda=True,
         max_iter=20, zero_tol=1e-6, optimizer=2, eta=0.001, gamma=0.01, beta1=0.9,
-        beta2=0.999, eps=1e-8, verbose=False, seed=42, init=None, precision="fp64"):
+        beta2=0.999, eps=1e-8, verbose=False, seed=42, init=None, precision="fp64",
+        checkpoint_every=0):
     """Wasserstein Dictionary Learning with the argument contract of wdl_cpp
     (src/wdl.cpp:108-170): Y (N x M) normalised documents, C (N x N) cost.
 
     Returns dict(A=N x S topics, W=S x M weights, Yhat=N x M barycenters).
     With init=(Alpha0, Lambda0) the explicit starting parameters replace the
     seeded draw (the reference's CPU path draws them with R's RNG);
-    precision "fp32" runs the Gibbs products and recurrences in FP32.
+    precision "fp32" runs the Gibbs products and recurrences in FP32;
+    checkpoint_every k > 0 keeps every k-th iterate for the adjoint and
+    recomputes the others (same results, less memory; with init only).
     sinkhorn_mode selects the inference of Yhat only, as in the reference
     (src/wdl_impl.cpp:354-377): 1 batches the parallel barycenters of all
     documents on the GPU; 2 batches their log-domain barycenters (FP64, GPU,
@@ -288,6 +291,9 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
     if init is None:
         if precision != "fp64":
             raise ValueError("the seeded entry point runs in FP64")
+        if checkpoint_every:
+            raise ValueError("checkpoint_every needs init: the seeded entry point keeps the "
+                             "argument list of cuda_wdl")
         check(lib().rwig_wdl(_p(A), _p(W), Yp, _p(Y), _p(C), N, M, int(S), *args,
                              int(seed), int(bool(verbose))))
     else:
@@ -297,7 +303,7 @@ def wdl(Y, C, reg, S, n_threads=0, batch_size=64, epochs=2, sinkhorn_mode=1, use
             raise ValueError("init must be (Alpha0 N x S, Lambda0 S x M)")
         prec = {"fp64": 0, "fp32": 1}[precision]
         check(lib().rwig_wdl_init(_p(A), _p(W), Yp, _p(Y), _p(C), N, M, int(S), *args,
-                                  _p(Alpha0), _p(Lambda0), prec))
+                                  _p(Alpha0), _p(Lambda0), prec, int(checkpoint_every)))
     if sinkhorn_mode == 2:
         # every document's log barycenter in one call, C resident on the device
         check(lib().rwig_barycenter_log_batched(_p(Yhat), None, None, _p(A), _p(W), _p(C), N, N,

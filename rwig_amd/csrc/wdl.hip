@@ -10,7 +10,8 @@
 // -- is one MFMA GEMM of K (N x N) with an N x S*D matrix (FP64:
 // v_mfma_f64_16x16x4, FP32: v_mfma_f32_16x16x4), with the elementwise
 // recurrences fused into the GEMM epilogues. The forward keeps the iterates
-// of all L steps (as the reference does) for the adjoint.
+// of all L steps (as the reference does) for the adjoint, or only every
+// ck-th VB, the adjoint then recomputing one segment at a time.
 #include <algorithm>
 #include <random>
 
@@ -65,24 +66,36 @@ k_coldot(const T *A, const T *X, long ldx, int N, int S, double *out) {
 //   A (N x S), Yb (N x D), Wb (S x D): column-major, leading dims N, N, S
 // Outputs (FP64): gAlpha (N x S), glambda (S), loss = sum_d ||b_d - y_d||^2
 // of the unnormalised barycenters of the L-th iteration (may be null).
+// ck: 0 keeps the iterates of all L steps for the adjoint, as the reference
+// does (src/wdl_impl.cpp:111-121); 0 < ck < L keeps VB every ck iterations
+// and recomputes each segment's iterates (UB, KVB, KTUB, bB, VB) with the
+// forward's own kernels before stepping back through it: the same gradient
+// bit for bit, with ck + L / ck + 1 slices of history instead of L + 1, for
+// L more forward steps (2L more GEMMs).
 template <typename T>
 void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, int D,
                     int L, const T *A, const T *Yb, const T *Wb, double *gAlpha,
-                    double *glambda, double *loss) {
+                    double *glambda, double *loss, int ck) {
   RWIG_REQUIRE(L >= 1, "the WDL step needs at least one iteration");
+  RWIG_REQUIRE(ck >= 0, "checkpoint interval must be non-negative");
   const int SD = S * D;
   const long ldn = (N + 7) / 8 * 8;  // leading dimension of the N x SD work matrices
   const long MSD = ldn * SD;
+  const bool ckm = ck > 0 && ck < L;
+  const int SL = ckm ? ck : L;  // steps of history held at once
   WdlBuf<T> wb;
-  // histories (L+1 slices; KVBh uses L)
-  T *UBh = wb.make((size_t)(L + 1) * MSD), *VBh = wb.make((size_t)(L + 1) * MSD);
-  T *KVBh = wb.make((size_t)L * MSD), *KTUBh = wb.make((size_t)(L + 1) * MSD);
-  T *bBh = wb.make((size_t)(L + 1) * N * D);
-  auto UB = [=](int l) { return UBh + (long)l * MSD; };
-  auto VB = [=](int l) { return VBh + (long)l * MSD; };
-  auto KVB = [=](int l) { return KVBh + (long)l * MSD; };
-  auto KTUB = [=](int l) { return KTUBh + (long)l * MSD; };
-  auto bB = [=](int l) { return bBh + (long)l * N * D; };
+  // history slots: iterate l sits in slot l - base (base 0 without
+  // checkpoints; the first step of the current segment with them)
+  T *UBh = wb.make((size_t)(SL + 1) * MSD), *VBh = wb.make((size_t)(SL + 1) * MSD);
+  T *KVBh = wb.make((size_t)SL * MSD), *KTUBh = wb.make((size_t)(SL + 1) * MSD);
+  T *bBh = wb.make((size_t)(SL + 1) * N * D);
+  T *ckV = ckm ? wb.make((size_t)((L - 1) / ck + 1) * MSD) : nullptr;  // VB at l = 0, ck, ...
+  int base = 0;
+  auto UB = [&](int l) { return UBh + (long)(l - base) * MSD; };
+  auto VB = [&](int l) { return VBh + (long)(l - base) * MSD; };
+  auto KVB = [&](int l) { return KVBh + (long)(l - base) * MSD; };
+  auto KTUB = [&](int l) { return KTUBh + (long)(l - base) * MSD; };
+  auto bB = [&](int l) { return bBh + (long)(l - base) * N * D; };
 
   // VB_0 = UB_0 = 1 (padding rows 0), bB_0 = 0
   {
@@ -95,8 +108,8 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
     T *b0 = bB(0);
     launch_map(st, (long)N * D, [=] __device__(long k) { b0[k] = T(0); });
   }
-  // ------------------------------------------------------------ forward
-  for (int l = 0; l < L; ++l) {
+  // forward step l: VB_l -> KVB_l, UB_{l+1}, KTUB_{l+1}, bB_{l+1}, VB_{l+1}
+  auto fwd = [&](int l) {
     T *kvb = KVB(l), *ub1 = UB(l + 1), *ktub1 = KTUB(l + 1), *vb1 = VB(l + 1);
     T *bb1 = bB(l + 1);
     // KVB = K VB_l ; UB_{l+1} = A / KVB
@@ -119,6 +132,15 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
         vb1[kk] = p / ktub1[kk];
       }
     });
+  };
+  // ------------------------------------------------------------ forward
+  for (int l = 0; l < L; ++l) {
+    if (ckm && l - base == SL) {  // the slots are full: VB_l starts a segment
+      d2d(VBh, VB(l), (size_t)MSD, st);
+      base = l;
+    }
+    if (ckm && l % ck == 0) d2d(ckV + (long)(l / ck) * MSD, VB(l), (size_t)MSD, st);
+    fwd(l);
   }
   if (loss) {
     const T *bbL = bB(L);
@@ -141,6 +163,12 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
     Xb[k] = T(0);
   });
   for (int l = L; l > 0; --l) {
+    if (ckm && l == base) {
+      // step back into the previous segment: its iterates from its checkpoint
+      base = l - ck;
+      d2d(VB(base), ckV + (long)(base / ck) * MSD, (size_t)MSD, st);
+      for (int q = base; q < l; ++q) fwd(q);
+    }
     const T *ktubl = KTUB(l), *vbl = VB(l), *bbl = bB(l), *ubl = UB(l);
     const T *kvblm1 = KVB(l - 1);
     if (l == L) {
@@ -210,10 +238,10 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
 
 template void wdl_batch_grad<double>(hipStream_t, const double *, const double *, long, int, int,
                                      int, int, const double *, const double *, const double *,
-                                     double *, double *, double *);
+                                     double *, double *, double *, int);
 template void wdl_batch_grad<float>(hipStream_t, const float *, const float *, long, int, int,
                                     int, int, const float *, const float *, const float *,
-                                    double *, double *, double *);
+                                    double *, double *, double *, int);
 
 namespace {
 
@@ -333,7 +361,7 @@ template <typename T>
 void wdl_fit(hipStream_t st, const double *Y, const double *C, int N, int M, int S, double reg,
              int max_iter, double zero_tol, int Bsz, int E, int opt, double eta, double gamma,
              double beta1, double beta2, double eps, const double *Alpha0,
-             const double *Lambda0, double *A_out, double *W_out, double *Yhat_out) {
+             const double *Lambda0, double *A_out, double *W_out, double *Yhat_out, int ck) {
   const long ldk = (N + 7) / 8 * 8;
   DBuf<T> Kb(ldk * N), KTb(ldk * N), Ab((size_t)N * S), Wt((size_t)S * M), Yt((size_t)N * M);
   DBuf<double> dC((size_t)N * N), dY((size_t)N * M), Alpha((size_t)N * S),
@@ -372,7 +400,7 @@ void wdl_fit(hipStream_t st, const double *Y, const double *C, int N, int M, int
       cast_to<T>(st, Wd.get(), Wt.get(), (long)S * M);
       wdl_batch_grad<T>(st, Kb.get(), KTb.get(), ldk, N, S, D, max_iter, Ab.get(),
                         Yt.get() + (long)bid * Bsz * N, Wt.get() + (long)bid * Bsz * S,
-                        gA.get(), gl.get(), nullptr);
+                        gA.get(), gl.get(), nullptr, ck);
       // g_Lambda = g_lambda 1^T over all M documents (src/wdl_impl.cpp:246)
       {
         double *gLp = gL.get();
@@ -439,7 +467,7 @@ int rwig_wdl(double *A, double *W, double *Yhat, const double *Y, const double *
     for (auto &x : Lambda0) x = randn(rng);
     Stream s;
     wdl_fit<double>(s.get(), Y, C, N, M, S, reg, max_iter, zero_tol, B, E, opt, eta, gamma,
-                    beta1, beta2, eps, Alpha0.data(), Lambda0.data(), A, W, Yhat);
+                    beta1, beta2, eps, Alpha0.data(), Lambda0.data(), A, W, Yhat, 0);
     (void)verbose;
   });
 }
@@ -447,22 +475,25 @@ int rwig_wdl(double *A, double *W, double *Yhat, const double *Y, const double *
 // WDL fit from explicit initial Alpha0 (N x S) and Lambda0 (S x M), the
 // parameters the reference draws with R's RNG (src/wdl_impl.cpp:277-278).
 // precision 0: FP64, 1: FP32 GEMMs and recurrences (optimizer and softmax in
-// FP64).
+// FP64). checkpoint_every: 0 keeps every iterate for the adjoint, k > 0 every
+// k-th (same result, less memory, more GEMMs).
 int rwig_wdl_init(double *A, double *W, double *Yhat, const double *Y, const double *C, int N,
                   int M, int S, double reg, int max_iter, double zero_tol, int B, int E, int opt,
                   double eta, double gamma, double beta1, double beta2, double eps,
-                  const double *Alpha0, const double *Lambda0, int precision) {
+                  const double *Alpha0, const double *Lambda0, int precision,
+                  int checkpoint_every) {
   return guarded("rwig_wdl_init", [&] {
     wdl_check(N, M, S, reg, max_iter, B, E, opt);
     RWIG_REQUIRE(Y && C && Alpha0 && Lambda0, "Y, C, Alpha0 and Lambda0 are required");
+    RWIG_REQUIRE(checkpoint_every >= 0, "checkpoint_every must be non-negative");
     ensure_device();
     Stream s;
     if (precision == 0)
       wdl_fit<double>(s.get(), Y, C, N, M, S, reg, max_iter, zero_tol, B, E, opt, eta, gamma,
-                      beta1, beta2, eps, Alpha0, Lambda0, A, W, Yhat);
+                      beta1, beta2, eps, Alpha0, Lambda0, A, W, Yhat, checkpoint_every);
     else if (precision == 1)
       wdl_fit<float>(s.get(), Y, C, N, M, S, reg, max_iter, zero_tol, B, E, opt, eta, gamma,
-                     beta1, beta2, eps, Alpha0, Lambda0, A, W, Yhat);
+                     beta1, beta2, eps, Alpha0, Lambda0, A, W, Yhat, checkpoint_every);
     else
       throw Error(RWIG_ERR_INVALID, "precision must be 0 (fp64) or 1 (fp32)");
   });
@@ -473,22 +504,25 @@ int rwig_wdl_init(double *A, double *W, double *Yhat, const double *Y, const dou
 // a multiple of 8 elements), A (N x S), Yb (N x D), Wb (S x D) in the
 // precision given (0: double, 1: float); gAlpha (N x S), glambda (S) and
 // loss (sum of squared residuals of the unnormalised b_L, may be null) FP64.
+// checkpoint_every as in rwig_wdl_init.
 int rwig_wdl_batch_grad_dev(const void *K, const void *KT, long ldk, int N, int S, int D, int L,
                             const void *A, const void *Yb, const void *Wb, double *gAlpha,
-                            double *glambda, double *loss, int precision, void *stream) {
+                            double *glambda, double *loss, int precision, int checkpoint_every,
+                            void *stream) {
   return guarded("rwig_wdl_batch_grad_dev", [&] {
     RWIG_REQUIRE(K && KT && A && Yb && Wb && gAlpha && glambda, "null argument");
     RWIG_REQUIRE(N > 0 && S > 0 && D > 0 && L > 0 && ldk >= N && ldk % 8 == 0,
                  "invalid dimensions");
+    RWIG_REQUIRE(checkpoint_every >= 0, "checkpoint_every must be non-negative");
     hipStream_t st = (hipStream_t)stream;
     if (precision == 0)
       wdl_batch_grad<double>(st, (const double *)K, (const double *)KT, ldk, N, S, D, L,
                              (const double *)A, (const double *)Yb, (const double *)Wb, gAlpha,
-                             glambda, loss);
+                             glambda, loss, checkpoint_every);
     else if (precision == 1)
       wdl_batch_grad<float>(st, (const float *)K, (const float *)KT, ldk, N, S, D, L,
                             (const float *)A, (const float *)Yb, (const float *)Wb, gAlpha,
-                            glambda, loss);
+                            glambda, loss, checkpoint_every);
     else
       throw Error(RWIG_ERR_INVALID, "precision must be 0 (fp64) or 1 (fp32)");
   });

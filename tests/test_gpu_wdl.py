@@ -59,7 +59,7 @@ def test_batch_grad_matches_oracle(precision, tol):
     check(lib().rwig_wdl_batch_grad_dev(dev[0].data_ptr(), dev[1].data_ptr(), ldk, N, S, D, L,
                                         At.data_ptr(), Yt.data_ptr(), Wt.data_ptr(),
                                         gA.data_ptr(), gl.data_ptr(), loss.data_ptr(),
-                                        precision, torch.cuda.current_stream().cuda_stream))
+                                        precision, 0, torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     assert rel(gA.cpu().numpy().T, gA_o) < tol
     assert rel(gl.cpu().numpy(), gl_o) < tol
@@ -144,7 +144,7 @@ class C4Batch:
             self.dev[prec] = (Kt.cuda(), KTt.cuda(),
                               torch.from_numpy(Y.T.copy()).to(dt).cuda())
 
-    def grad(self, Alpha, Lambda, precision=0):
+    def grad(self, Alpha, Lambda, precision=0, ck=0):
         Kd, KTd, Yd = self.dev[precision]
         dt = Yd.dtype
         A = O.softmax_cols(Alpha)
@@ -157,7 +157,7 @@ class C4Batch:
         check(lib().rwig_wdl_batch_grad_dev(Kd.data_ptr(), KTd.data_ptr(), self.ldk, self.N,
                                             self.S, self.D, self.L, At.data_ptr(),
                                             Yd.data_ptr(), Wt.data_ptr(), gA.data_ptr(),
-                                            gl.data_ptr(), loss.data_ptr(), precision,
+                                            gl.data_ptr(), loss.data_ptr(), precision, ck,
                                             torch.cuda.current_stream().cuda_stream))
         torch.cuda.synchronize()
         return gA.cpu().numpy().T, gl.cpu().numpy(), loss.item()
@@ -206,3 +206,61 @@ def test_c4_fp32_against_fp64(c4batch):
     eloss = abs(l32 - l64) / abs(l64)
     print(f"C4 FP32 vs FP64: g_Alpha {eA:.2e}, g_lambda {el:.2e}, loss {eloss:.2e}", flush=True)
     assert max(eA, el, eloss) < TOL_C4_FP32
+
+
+# ------------------------------------------------------ checkpointed adjoint
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_checkpointed_batch_grad_is_bit_identical(precision):
+    # the adjoint's segments are recomputed with the forward's own kernels
+    N, S, D, L, reg = 70, 4, 6, 12, 0.3
+    C, Y, rng = wdl_problem(6, N, D, S)
+    K = np.exp(-C / reg)
+    A = O.softmax_cols(rng.standard_normal((N, S)))
+    Wb = O.softmax_cols(rng.standard_normal((S, D)))
+    dt = torch.float64 if precision == 0 else torch.float32
+    ldk = (N + 7) // 8 * 8
+    Kt = torch.zeros((N, ldk), dtype=dt)
+    Kt[:, :N] = torch.from_numpy(K.T.copy())
+    KTt = torch.zeros((N, ldk), dtype=dt)
+    KTt[:, :N] = torch.from_numpy(K.copy())
+    Kd, KTd = Kt.cuda(), KTt.cuda()
+    At = torch.from_numpy(A.T.copy()).to(dt).cuda()
+    Yt = torch.from_numpy(Y.T.copy()).to(dt).cuda()
+    Wt = torch.from_numpy(Wb.T.copy()).to(dt).cuda()
+    outs = []
+    for ck in [0, 1, 3, 5, 7, 11, 12, 40]:
+        gA = torch.zeros((S, N), dtype=torch.float64, device="cuda")
+        gl = torch.zeros(S, dtype=torch.float64, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        check(lib().rwig_wdl_batch_grad_dev(Kd.data_ptr(), KTd.data_ptr(), ldk, N, S, D, L,
+                                            At.data_ptr(), Yt.data_ptr(), Wt.data_ptr(),
+                                            gA.data_ptr(), gl.data_ptr(), loss.data_ptr(),
+                                            precision, ck, torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        outs.append((ck, gA.cpu(), gl.cpu(), loss.cpu()))
+    for ck, gA, gl, loss in outs[1:]:
+        assert torch.equal(gA, outs[0][1]), ck
+        assert torch.equal(gl, outs[0][2]), ck
+        assert torch.equal(loss, outs[0][3]), ck
+
+
+def test_checkpointed_fit_is_bit_identical():
+    N, M, S = 60, 12, 3
+    C, Y, rng = wdl_problem(7, N, M, S)
+    init = (rng.standard_normal((N, S)), rng.standard_normal((S, M)))
+    kw = dict(batch_size=5, epochs=2, max_iter=10, optimizer=1, eta=0.01, init=init)
+    r0 = rwig_amd.wdl(Y, C, 0.3, S, **kw)
+    r4 = rwig_amd.wdl(Y, C, 0.3, S, checkpoint_every=4, **kw)
+    for k in ["A", "W", "Yhat"]:
+        assert np.array_equal(r0[k], r4[k]), k
+
+
+def test_c4_checkpointed_gradient_is_bit_identical(c4batch):
+    b = c4batch
+    for prec in (0, 1):
+        full = b.grad(b.Alpha0, b.Lambda0, prec, ck=0)
+        ck5 = b.grad(b.Alpha0, b.Lambda0, prec, ck=5)
+        assert np.array_equal(full[0], ck5[0]) and np.array_equal(full[1], ck5[1])
+        assert full[2] == ck5[2]

@@ -256,7 +256,6 @@ k_sinkhorn_vanilla_tiny(const double *__restrict__ a, const double *__restrict__
   __shared__ double red[kTinyWaves];
   __shared__ int active;
   const int tid = threadIdx.x;
-  const long MN = (long)M * N;
   for (int j = tid; j < N; j += kTinyTB) sv[j] = 1.0;
   tiny_row_pass(K, M, N, sv, sKv, part);
   if (uh) {

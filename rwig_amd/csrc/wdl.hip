@@ -58,6 +58,25 @@ k_coldot(const T *A, const T *X, long ldx, int N, int S, double *out) {
   if (threadIdx.x == 0) out[c] = a;
 }
 
+// glambda(s) = mean over d of w(s,d) (wbar(s,d) - sum_q w(q,d) wbar(q,d)), the
+// softmax Jacobian of the weights; one block per atom s, the documents split
+// over its threads (fixed order)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_lambda_grad(const T *Wb, const double *wB, int S, int D, double *glambda) {
+  __shared__ double sh[256 / kWave];
+  const int s = blockIdx.x;
+  double a = 0.0;
+  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+    double dw = 0.0;
+    for (int q = 0; q < S; ++q) dw += (double)Wb[q + (long)d * S] * wB[(long)d * S + q];
+    const double ws = (double)Wb[s + (long)d * S];
+    a += ws * wB[(long)d * S + s] - ws * dw;
+  }
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) glambda[s] = a / D;
+}
+
 }  // namespace
 
 // One batched WDL gradient (src/wdl_impl.cpp:87-247).
@@ -224,16 +243,8 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
   });
   hipLaunchKernelGGL((k_colsum_to<T, double>), dim3(SD), dim3(256), 0, st, Tw, ldn, N, wB);
   check_launch();
-  launch_map(st, (long)S, [=] __device__(long s) {
-    double acc = 0.0;
-    for (int d = 0; d < D; ++d) {
-      double dw = 0.0;
-      for (int q = 0; q < S; ++q) dw += (double)Wb[q + d * S] * wB[d * S + q];
-      const double ws = (double)Wb[s + d * S];
-      acc += ws * wB[d * S + s] - ws * dw;
-    }
-    glambda[s] = acc / D;
-  });
+  hipLaunchKernelGGL(k_lambda_grad<T>, dim3(S), dim3(256), 0, st, Wb, wB, S, D, glambda);
+  check_launch();
 }
 
 template void wdl_batch_grad<double>(hipStream_t, const double *, const double *, long, int, int,

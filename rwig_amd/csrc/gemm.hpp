@@ -427,15 +427,26 @@ __device__ __forceinline__ void gemm_main_regs(
   }
 }
 
+// s_waitcnt lgkmcnt(0) with the other counters left alone
+__device__ __forceinline__ void wait_lgkmcnt0() {
+  __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14));
+}
+
 // Main loop, direct-to-LDS path (CF::DMA): each global_load_lds_dwordx4
 // moves 1 KB (64 lanes x 16 bytes, written to LDS in lane order): 8 tile rows
 // of 64 k or 4 of 128 k. The lane with slot s of tile row r loads the 16-byte
 // k chunk s ^ (r % 8), so that chunk c of row r sits at slot c ^ (r % 8) and
 // the MFMA fragment reads (16 rows, one chunk) spread over all LDS banks.
 // Rows past the matrix read a valid row instead: they only feed outputs the
-// epilogue discards. PF + 1 stages; the barrier of step s follows this
-// wavefront's wait for its own moves of step s (the moves of later steps stay
-// in flight), and then step s + PF reuses the stage of step s - 1.
+// epilogue discards.
+//
+// PF + 1 LDS stages and two register sets of fragments. Step t: wait for this
+// wavefront's moves of step t + 1 (those of later steps stay in flight) and
+// for its fragment reads of step t, one barrier, then the moves of step
+// t + PF + 1 into the stage of step t (every wavefront holds step t's
+// fragments in registers by then), the fragment reads of step t + 1 into the
+// other set, and the MFMAs of step t, which the reads overlap. Each output
+// element sees the MFMAs of the register path in the same order.
 template <typename T, class CF, class Acc>
 __device__ __forceinline__ void gemm_main_dma(
     int M, int N, const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb,
@@ -452,7 +463,8 @@ __device__ __forceinline__ void gemm_main_dma(
   constexpr int STAGEB = (BM + BN) * ROWB;
   constexpr int MA = BM / NW / RPM, MB = BN / NW / RPM;  // moves per wavefront and k-step
   constexpr int MOVES = MA + MB;
-  static_assert((PF - 1) * MOVES < 64, "the moves in flight must fit vmcnt");
+  constexpr int KS = BK / MF::KI;  // MFMA k-slices per step
+  static_assert(PF * MOVES < 64, "the moves in flight must fit vmcnt");
   __shared__ __attribute__((aligned(1024))) char smem[STAGES * STAGEB];
   RWIG_LDS char *lds = (RWIG_LDS char *)smem;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -473,6 +485,7 @@ __device__ __forceinline__ void gemm_main_dma(
     pb[q] = B + r * ldb + (slot ^ (rl & 7)) * EPC;
   }
   const int nst = (kend - kbeg) / BK;
+  if (nst <= 0) return;
   auto issue = [&](int s) {
     RWIG_LDS char *st = lds + (s % STAGES) * STAGEB;
     const int k = kbeg + s * BK;
@@ -483,9 +496,6 @@ __device__ __forceinline__ void gemm_main_dma(
     for (int q = 0; q < MB; ++q)
       lds_move16(pb[q] + k, (RWIG_LDS void *)(st + (BM + (wv * MB + q) * RPM) * ROWB));
   };
-#pragma unroll
-  for (int q = 0; q < PF; ++q)
-    if (q < nst) issue(q);
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;  // the lane's row and first k
   // fragment of tile row `row` at k: chunk k / EPC (at its swizzled slot),
   // element k % EPC within it
@@ -493,29 +503,75 @@ __device__ __forceinline__ void gemm_main_dma(
     return *reinterpret_cast<const RWIG_LDS typename MF::frag_t *>(
         tile + row * ROWB + (((k / EPC) ^ (row & 7)) << 4) + (k % EPC) * (int)sizeof(T));
   };
-  for (int s = 0; s < nst; ++s) {
-    // this wavefront's moves of step s have landed once only the later
-    // steps' moves are outstanding
-    const int later = min(nst, s + PF) - (s + 1);
-    wait_later<MOVES, PF - 1>(later);
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    if (s + PF < nst) issue(s + PF);
+  typename MF::frag_t fa[2][KS][TM], fb[2][KS][TN];
+  auto read = [&](int set, int s) {
     const RWIG_LDS char *as = lds + (s % STAGES) * STAGEB, *bs = as + BM * ROWB;
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += MF::KI) {
-      const int k = kk + lk;
-      typename MF::frag_t af[TM], bfr[TN];
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * MF::KI + lk;
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = frag(as, wm * WM + i * 16 + lr, k);
+      for (int i = 0; i < TM; ++i) fa[set][ks][i] = frag(as, wm * WM + i * 16 + lr, k);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bfr[j] = frag(bs, wn * WN + j * 16 + lr, k);
+      for (int j = 0; j < TN; ++j) fb[set][ks][j] = frag(bs, wn * WN + j * 16 + lr, k);
+    }
+  };
+  auto mma = [&](int set) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], af[i], bfr[j]);
+        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], fa[set][ks][i], fb[set][ks][j]);
+  };
+  auto barrier = [] {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+#pragma unroll
+  for (int q = 0; q < STAGES; ++q)
+    if (q < nst) issue(q);
+  wait_later<MOVES, PF>(min(nst, STAGES) - 1);
+  barrier();
+  read(0, 0);
+  // Steps away from the end take compile-time waits and no branches: with no
+  // join between the reads of step t + 1 and the MFMAs of step t, the
+  // compiler does not wait for those reads before the MFMAs. The scheduling
+  // barriers keep the reads ahead of the MFMAs and the MFMAs ahead of the
+  // next step's wait.
+  auto steady = [&](auto set_c, int t) {
+    constexpr int SET = decltype(set_c)::value;
+    wait_vmcnt<(PF - 1) * MOVES>();
+    wait_lgkmcnt0();
+    barrier();
+    issue(t + STAGES);
+    read(SET ^ 1, t + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(SET);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto last = [&](auto set_c, int t) {
+    constexpr int SET = decltype(set_c)::value;
+    if (t >= nst) return;
+    if (t + 1 < nst) {
+      wait_later<MOVES, PF - 1>(min(nst, t + STAGES) - (t + 2));
+      wait_lgkmcnt0();
+      barrier();
+      if (t + STAGES < nst) issue(t + STAGES);
+      read(SET ^ 1, t + 1);
     }
+    mma(SET);
+  };
+  using Set0 = std::integral_constant<int, 0>;
+  using Set1 = std::integral_constant<int, 1>;
+  int t = 0;
+  for (; t + 1 + STAGES < nst; t += 2) {
+    steady(Set0{}, t);
+    steady(Set1{}, t + 1);
+  }
+  for (; t < nst; t += 2) {
+    last(Set0{}, t);
+    last(Set1{}, t + 1);
   }
 }
 

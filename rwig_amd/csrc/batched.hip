@@ -46,6 +46,9 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
 
   const int t = blockIdx.x;
   const int c0 = col_ptr[t], nt = col_ptr[t + 1] - c0;
+  // exp(-x / reg) of the per-entry terms as exp2(-x * sc): one multiply in
+  // place of an FP64 division per term
+  const double sc = kLog2e / reg;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
 
   double f[RPT], rmin[RPT];
@@ -115,7 +118,7 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
           double sacc = 0.0;
 #pragma unroll
           for (int k = 0; k < RPT; ++k)
-            if (own[k]) sacc += exp(-(((c[k][q] - f[k]) - gq[q]) - mn) / reg);
+            if (own[k]) sacc += exp2_f64(-(((c[k][q] - f[k]) - gq[q]) - mn) * sc);
           sacc = wave_sum(sacc);
           if (lane == 0) red[kBW * CB + wv * CB + q] = sacc;
         }
@@ -153,10 +156,10 @@ k_sinkhorn_log_resident(const double *__restrict__ C, long ldc, int M,
 #pragma unroll
         for (int q = 0; q < CB; ++q)
           if (jb + q < nt) m = fmin(m, (c[k][q] - f[k]) - gq[q]);
-        double s = (m == rmn[k]) ? rsm[k] : rsm[k] * exp(-(rmn[k] - m) / reg);
+        double s = (m == rmn[k]) ? rsm[k] : rsm[k] * exp2_f64(-(rmn[k] - m) * sc);
 #pragma unroll
         for (int q = 0; q < CB; ++q)
-          if (jb + q < nt) s += exp(-(((c[k][q] - f[k]) - gq[q]) - m) / reg);
+          if (jb + q < nt) s += exp2_f64(-(((c[k][q] - f[k]) - gq[q]) - m) * sc);
         rmn[k] = m;
         rsm[k] = s;
       }

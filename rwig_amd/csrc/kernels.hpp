@@ -12,6 +12,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "exp2_f64.hpp"
 
 namespace rwig {
 
@@ -206,12 +207,12 @@ void gemv_t(hipStream_t st, const double *K, int M, int N, long ldk,
 // slices, F, G and the shifts once per row or column), so that
 // exp(-r / reg) = exp2(-r * sc) needs no multiply per (entry, slice). A
 // result m2 - log2(t) converts back with 1 / sc = reg ln 2.
-// exp2: FP64 keeps the library exp2 (the FP64 forms match the oracle to
-// 1e-10); FP32 uses the hardware v_exp_f32 (about 1 ulp; the arguments are
-// <= 0 and terms below 2^-126 vanish either way).
+// exp2: FP64 uses exp2_f64 (range reduction and a degree-11 polynomial,
+// about 1 ulp, straight-line); FP32 uses the hardware v_exp_f32 (about 1 ulp;
+// the arguments are <= 0 and terms below 2^-126 vanish either way).
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kLn2 = 0.69314718055994530942;
-__device__ __forceinline__ double sm_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ double sm_exp2(double x) { return exp2_f64(x); }
 __device__ __forceinline__ float sm_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 // exp2 of an argument held in T, evaluated in X: X = float with T = double is
 // the mixed form (FP64 arguments, the hardware FP32 exponential). The

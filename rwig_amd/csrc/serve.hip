@@ -398,13 +398,17 @@ struct Group {
   const int *d_lim() const { return lists.get() + 4 + 4 * cap; }
 
   // Runs one phase of a cycle on the stream: the list upload, then the
-  // absorption (phase 0) or absorb_every iterations (phase 1). With graphs,
-  // the phase is recorded once per column capacity and replayed: every
-  // launch reads its column counts and lists from device memory.
+  // absorption (phase 0) or absorb_every iterations (phase 1), then for the
+  // absorption the download of its stop flags and errors. With graphs, the
+  // kernels of the phase are recorded once per column capacity and replayed:
+  // every launch reads its column counts and lists from device memory. The
+  // copies between page-locked host buffers and the device stay outside the
+  // recorded graphs, as plain copies on the same stream before and after the
+  // replay, so the graphs hold kernel nodes only.
   void run_phase(int phase) {
     hipStream_t st = stream->get();
+    upload_lists(phase);
     auto body = [&] {
-      upload_lists(phase);
       if (phase == 0)
         absorb();
       else
@@ -430,7 +434,10 @@ struct Group {
       }
       RWIG_HIP_CHECK(hipGraphLaunch(gx[phase], st));
     }
-    if (phase == 0) RWIG_HIP_CHECK(hipEventRecord(ev, st));
+    if (phase == 0) {
+      download_flags();
+      RWIG_HIP_CHECK(hipEventRecord(ev, st));
+    }
   }
 
   // Column count for the tile policy: at most 32 columns take the 32 x 32
@@ -605,7 +612,11 @@ struct Group {
       const long t = act[c];
       if (!dn[t]) lvp[j + t * Ni] = md[t] ? 0.0f : 1.0f;
     });
-    d2h(hdone_pin, dn, (size_t)cap, st);
+  }
+  // the absorption's stop flags and errors into page-locked host buffers
+  void download_flags() {
+    hipStream_t st = stream->get();
+    d2h(hdone_pin, done.p<int>(), (size_t)cap, st);
     d2h(herr_pin, errs.p<double>(), (size_t)cap, st);
   }
 

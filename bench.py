@@ -276,9 +276,10 @@ def lse_leg(torch, dev, stream, log):
 
 def c4_leg(torch, dev, stream, log, cpu=False):
     """WDL training step (V = 2000, K = 16 atoms, 512 documents, eps 0.1, L = 20):
-    one batched gradient over all documents (the step's dominant work; the Adam
-    update is O(V K + K D)), FP32 and FP64, inputs resident in HBM; each also
-    with the adjoint's history checkpointed every 5 iterations."""
+    the batched gradient over all documents alone, and the whole step
+    (rwig_wdl_step_dev: softmax, gradient, Adam lr 1e-3 on Alpha and Lambda),
+    FP32 and FP64, inputs resident in HBM; each also with the adjoint's history
+    checkpointed every 5 iterations."""
     from rwig_amd._lib import check, lib
     from rwig_amd.workloads import c4
     C, Y, Alpha0, Lambda0, eps, L = c4()
@@ -293,7 +294,8 @@ def c4_leg(torch, dev, stream, log, cpu=False):
     A = softmax_cols(Alpha0)
     Wb = softmax_cols(Lambda0)
     ldk = (V + 7) // 8 * 8
-    out = {"workload": f"WDL batched gradient, V={V}, K={S} atoms, {D} documents, eps={eps}, "
+    out = {"workload": f"WDL training step (batched gradient; whole step with Adam lr 1e-3), "
+                       f"V={V}, K={S} atoms, {D} documents, eps={eps}, "
                        f"L={L}"}
     def history_bytes(ck, size):
         # adjoint history of the step: UB, VB, KTUB (SL + 1 slices), KVB (SL),
@@ -338,6 +340,32 @@ def c4_leg(torch, dev, stream, log, cpu=False):
         out[f"{key}_history_bytes"] = hb
         log(f"C4 {key}: {ms:.2f} ms per batched gradient ({D / ms * 1e3:.0f} documents/s), "
             f"adjoint history {hb / 2**30:.2f} GiB")
+        # the whole training step: softmax of Alpha and Lambda, the batched
+        # gradient, g_Lambda, Adam (lr 1e-3) on Alpha and Lambda
+        Al = torch.from_numpy(Alpha0.T.copy()).to(dev)
+        La = torch.from_numpy(Lambda0.T.copy()).to(dev)
+        mom = [torch.zeros_like(Al), torch.zeros_like(Al), torch.zeros_like(La),
+               torch.zeros_like(La)]
+        nstep = [0]
+
+        def train_step():
+            nstep[0] += 1
+            check(lib().rwig_wdl_step_dev(Kd.data_ptr(), KTd.data_ptr(), ldk, Yt.data_ptr(), V, D,
+                                          S, L, 0, D, Al.data_ptr(), La.data_ptr(),
+                                          *[m.data_ptr() for m in mom], nstep[0], 1, 1e-3, 0.01,
+                                          0.9, 0.999, 1e-8, loss.data_ptr(), p, ck,
+                                          stream.cuda_stream))
+        train_step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            train_step()
+        torch.cuda.synchronize()
+        ms_t = (time.perf_counter() - t) / reps * 1e3
+        out[f"{key}_train_step_ms"] = ms_t
+        out[f"{key}_train_docs_per_s"] = D / ms_t * 1e3
+        log(f"C4 {key}: {ms_t:.2f} ms per training step with Adam ({D / ms_t * 1e3:.0f} "
+            "documents/s)")
     if cpu:
         from oracle import oracle as O
         d = 2

@@ -177,6 +177,25 @@ int rwig_wdl_batch_grad_dev(const void *K, const void *KT, long ldk, int N,
                             double *glambda, double *loss, int precision,
                             int checkpoint_every, void *stream);
 
+/* One WDL training step on device buffers, the body of the reference's batch
+ * loop (src/wdl_impl.cpp:316-337, WDL::_train_batch_batched plus the
+ * optimizer step and softmax): A = softmax(Alpha), W = softmax(Lambda) by
+ * columns; the batched gradient of documents [first, first + D) of Y (as
+ * rwig_wdl_batch_grad_dev); g_Lambda = g_lambda 1^T over all M documents;
+ * optimizer step `step` (>= 1; optimizer 0 SGD, 1 Adam, 2 AdamW) on Alpha
+ * (N x S) and Lambda (S x M) with their moments mA, vA (N x S) and mL, vL
+ * (S x M), updated in place (zero before step 1). K, KT (leading dimension
+ * ldk) and Y (N x M) in the precision given (0 double, 1 float); Alpha,
+ * Lambda, the moments and loss (may be null) FP64. The sequence of steps over
+ * the batches of E epochs gives rwig_wdl_init's A and W bit for bit. */
+int rwig_wdl_step_dev(const void *K, const void *KT, long ldk, const void *Y,
+                      int N, int M, int S, int L, int first, int D,
+                      double *Alpha, double *Lambda, double *mA, double *vA,
+                      double *mL, double *vL, int step, int optimizer,
+                      double eta, double gamma, double beta1, double beta2,
+                      double eps, double *loss, int precision,
+                      int checkpoint_every, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batched solves, device pointers (no counterpart in the reference, whose GPU
  * path solves one problem per call, inst/include/cuda_interface.cuh:13-39; the

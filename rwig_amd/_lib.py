@@ -56,6 +56,8 @@ SIGNATURES = {
                        _d, _d, _dp, _dp, _i, _i], _i),
     "rwig_wdl_batch_grad_dev": ([_vp, _vp, ctypes.c_long, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _i, _i, _vp], _i),
+    "rwig_wdl_step_dev": ([_vp, _vp, ctypes.c_long, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp,
+                           _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _vp, _i, _i, _vp], _i),
     "rwig_gibbs_lowp_dev": ([_vp, ctypes.c_long, _i, _i, _d, _i, _vp, ctypes.c_long, _vp], _i),
     "rwig_euclidean": ([_dp, _i, _i, _dp, _i], _i),
     "rwig_euclidean_dev": ([_vp, _i, _i, ctypes.c_long, _vp, ctypes.c_long, _i, _vp], _i),

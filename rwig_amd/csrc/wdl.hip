@@ -58,6 +58,43 @@ k_coldot(const T *A, const T *X, long ldx, int N, int S, double *out) {
   if (threadIdx.x == 0) out[c] = a;
 }
 
+// Inference stopping test of document d = blockIdx.x (one block per
+// document, fixed order): err = ||U .* KV - A||_F over its S columns; the
+// document stops when err < tol or at the last iteration.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_infer_err(const T *U, const T *KV, long ldn, const T *A, int N, int S, int itp1,
+            int maxiter, double tol, int *done, double *errv) {
+  __shared__ double sh[256 / kWave];
+  const int d = blockIdx.x;
+  if (done[d]) return;  // the same for every thread of the block
+  double acc = 0.0;
+  for (int s = 0; s < S; ++s)
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const long k = i + ((long)d * S + s) * ldn;
+      const double r = (double)U[k] * (double)KV[k] - (double)A[i + (long)s * N];
+      acc += r * r;
+    }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    errv[d] = sqrt(acc);
+    if (!(itp1 < maxiter && errv[d] >= tol)) done[d] = 1;
+  }
+}
+
+// Yhat(:, d) = b(:, d) / sum b(:, d), one block per document
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_infer_normalise(const T *bb, int N, double *Yhat) {
+  __shared__ double sh[256 / kWave];
+  const int d = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a += (double)bb[i + (long)d * N];
+  const double tot = block_sum(a, sh);
+  for (int i = threadIdx.x; i < N; i += blockDim.x)
+    Yhat[i + (long)d * N] = (double)bb[i + (long)d * N] / tot;
+}
+
 // glambda(s) = mean over d of w(s,d) (wbar(s,d) - sum_q w(q,d) wbar(q,d)), the
 // softmax Jacobian of the weights; one block per atom s, the documents split
 // over its threads (fixed order)
@@ -271,8 +308,29 @@ __global__ void k_softmax_cols(const double *X, double *Y, int m, int n) {
   for (int i = 0; i < m; ++i) y[i] = y[i] / tot;
 }
 
+// the same for long columns (Alpha's N entries): one block per column
+__global__ void __launch_bounds__(256) k_softmax_cols_block(const double *X, double *Y, int m) {
+  __shared__ double sh[256 / kWave];
+  const double *x = X + (long)blockIdx.x * m;
+  double *y = Y + (long)blockIdx.x * m;
+  double mx = -INFINITY;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) mx = fmax(mx, x[i]);
+  mx = -block_min(-mx, sh);
+  double tot = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double e = exp(x[i] - mx);
+    y[i] = e;
+    tot += e;
+  }
+  tot = block_sum(tot, sh);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) y[i] = y[i] / tot;
+}
+
 void softmax_cols(hipStream_t st, const double *X, double *Y, int m, int n) {
-  hipLaunchKernelGGL(k_softmax_cols, dim3((n + 255) / 256), dim3(256), 0, st, X, Y, m, n);
+  if (m >= 256)
+    hipLaunchKernelGGL(k_softmax_cols_block, dim3(n), dim3(256), 0, st, X, Y, m);
+  else
+    hipLaunchKernelGGL(k_softmax_cols, dim3((n + 255) / 256), dim3(256), 0, st, X, Y, m, n);
   check_launch();
 }
 
@@ -345,26 +403,51 @@ void wdl_infer(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, 
       if (!done[c / S]) KV[i + (long)c * ldn] = v;
     });
     // per-document err = ||U .* KV - A||_F ; stop when err < tol
-    const int itp1 = it + 1;
-    launch_map(st, (long)Dn, [=] __device__(long d) {
-      if (done[d]) return;
-      double acc = 0.0;
-      for (int s = 0; s < S; ++s)
-        for (int i = 0; i < N; ++i) {
-          const long k = i + (d * S + s) * ldn;
-          const double r = (double)U[k] * (double)KV[k] - (double)A[i + s * N];
-          acc += r * r;
-        }
-      errv[d] = sqrt(acc);
-      if (!(itp1 < maxiter && errv[d] >= tol)) done[d] = 1;
-    });
+    hipLaunchKernelGGL(k_infer_err<T>, dim3(Dn), dim3(256), 0, st, U, KV, ldn, A, N, S, it + 1,
+                       maxiter, tol, done, errv);
+    check_launch();
   }
   // b / sum(b)
-  launch_map(st, (long)Dn, [=] __device__(long d) {
-    double tot = 0.0;
-    for (int i = 0; i < N; ++i) tot += (double)bb[i + d * N];
-    for (int i = 0; i < N; ++i) Yhat[i + d * N] = (double)bb[i + d * N] / tot;
-  });
+  hipLaunchKernelGGL(k_infer_normalise<T>, dim3(Dn), dim3(256), 0, st, bb, N, Yhat);
+  check_launch();
+}
+
+// Buffers of one WDL step: the softmax images of Alpha (N x S) and Lambda
+// (S x M) in FP64 and in T, the gradients, and g_Lambda (S x M).
+template <typename T> struct WdlStepBufs {
+  DBuf<double> Ad, Wd, gA, gl, gL;
+  DBuf<T> Ab, Wt;
+  WdlStepBufs(int N, int S, int M)
+      : Ad((size_t)N * S), Wd((size_t)S * M), gA((size_t)N * S), gl(S), gL((size_t)S * M),
+        Ab((size_t)N * S), Wt((size_t)S * M) {}
+};
+
+// One step of the batch loop (src/wdl_impl.cpp:316-337): A = softmax(Alpha),
+// W = softmax(Lambda); the batched gradient of documents [first, first + D)
+// of Y (N x M in T); g_Lambda = g_lambda 1^T over all M documents
+// (src/wdl_impl.cpp:246); the optimizer step `step` (1-based) on Alpha and
+// Lambda with their moments (mA, vA: N x S; mL, vL: S x M).
+template <typename T>
+void wdl_step(hipStream_t st, const T *K, const T *KT, long ldk, const T *Y, int N, int M,
+              int S, int L, int first, int D, double *Alpha, double *Lambda, double *mA,
+              double *vA, double *mL, double *vL, int step, int opt, double eta, double gamma,
+              double beta1, double beta2, double eps, double *loss, int ck,
+              WdlStepBufs<T> &b) {
+  softmax_cols(st, Alpha, b.Ad.get(), N, S);
+  softmax_cols(st, Lambda, b.Wd.get(), S, M);
+  cast_to<T>(st, b.Ad.get(), b.Ab.get(), (long)N * S);
+  cast_to<T>(st, b.Wd.get() + (long)first * S, b.Wt.get(), (long)S * D);
+  wdl_batch_grad<T>(st, K, KT, ldk, N, S, D, L, b.Ab.get(), Y + (long)first * N, b.Wt.get(),
+                    b.gA.get(), b.gl.get(), loss, ck);
+  {
+    double *gLp = b.gL.get();
+    const double *glp = b.gl.get();
+    launch_map(st, (long)S * M, [=] __device__(long k) { gLp[k] = glp[k % S]; });
+  }
+  optimizer_step(st, opt, Alpha, b.gA.get(), mA, vA, (long)N * S, step, eta, gamma, beta1,
+                 beta2, eps);
+  optimizer_step(st, opt, Lambda, b.gL.get(), mL, vL, (long)S * M, step, eta, gamma, beta1,
+                 beta2, eps);
 }
 
 // The whole WDL fit (src/wdl_impl.cpp:254-378) from explicit Alpha0, Lambda0.
@@ -374,11 +457,11 @@ void wdl_fit(hipStream_t st, const double *Y, const double *C, int N, int M, int
              double beta1, double beta2, double eps, const double *Alpha0,
              const double *Lambda0, double *A_out, double *W_out, double *Yhat_out, int ck) {
   const long ldk = (N + 7) / 8 * 8;
-  DBuf<T> Kb(ldk * N), KTb(ldk * N), Ab((size_t)N * S), Wt((size_t)S * M), Yt((size_t)N * M);
+  DBuf<T> Kb(ldk * N), KTb(ldk * N), Yt((size_t)N * M);
   DBuf<double> dC((size_t)N * N), dY((size_t)N * M), Alpha((size_t)N * S),
-      Lambda((size_t)S * M), Ad((size_t)N * S), Wd((size_t)S * M), gA((size_t)N * S),
-      gl(S), gL((size_t)S * M), mA((size_t)N * S), vA((size_t)N * S), mL((size_t)S * M),
+      Lambda((size_t)S * M), mA((size_t)N * S), vA((size_t)N * S), mL((size_t)S * M),
       vL((size_t)S * M), Yh((size_t)N * M);
+  WdlStepBufs<T> b(N, S, M);
   h2d(dC.get(), C, (size_t)N * N, st);
   h2d(dY.get(), Y, (size_t)N * M, st);
   h2d(Alpha.get(), Alpha0, (size_t)N * S, st);
@@ -399,34 +482,23 @@ void wdl_fit(hipStream_t st, const double *Y, const double *C, int N, int M, int
     double *p = pm[q];
     launch_map(st, pn[q], [=] __device__(long k) { p[k] = 0.0; });
   }
-  softmax_cols(st, Alpha.get(), Ad.get(), N, S);
-  softmax_cols(st, Lambda.get(), Wd.get(), S, M);
   const int batches = (M % Bsz) ? (M / Bsz + 1) : (M / Bsz);
   int step = 0;
   for (int e = 0; e < E; ++e) {
     for (int bid = 0; bid < batches; ++bid) {
       // the reference's batch size rule (src/wdl_impl.cpp:91)
       const int D = (bid == (M / Bsz)) ? (M % Bsz) : Bsz;
-      cast_to<T>(st, Ad.get(), Ab.get(), (long)N * S);
-      cast_to<T>(st, Wd.get(), Wt.get(), (long)S * M);
-      wdl_batch_grad<T>(st, Kb.get(), KTb.get(), ldk, N, S, D, max_iter, Ab.get(),
-                        Yt.get() + (long)bid * Bsz * N, Wt.get() + (long)bid * Bsz * S,
-                        gA.get(), gl.get(), nullptr, ck);
-      // g_Lambda = g_lambda 1^T over all M documents (src/wdl_impl.cpp:246)
-      {
-        double *gLp = gL.get();
-        const double *glp = gl.get();
-        launch_map(st, (long)S * M, [=] __device__(long k) { gLp[k] = glp[k % S]; });
-      }
       ++step;
-      optimizer_step(st, opt, Alpha.get(), gA.get(), mA.get(), vA.get(), (long)N * S, step,
-                     eta, gamma, beta1, beta2, eps);
-      optimizer_step(st, opt, Lambda.get(), gL.get(), mL.get(), vL.get(), (long)S * M, step,
-                     eta, gamma, beta1, beta2, eps);
-      softmax_cols(st, Alpha.get(), Ad.get(), N, S);
-      softmax_cols(st, Lambda.get(), Wd.get(), S, M);
+      wdl_step<T>(st, Kb.get(), KTb.get(), ldk, Yt.get(), N, M, S, max_iter, bid * Bsz, D,
+                  Alpha.get(), Lambda.get(), mA.get(), vA.get(), mL.get(), vL.get(), step, opt,
+                  eta, gamma, beta1, beta2, eps, nullptr, ck, b);
     }
   }
+  // A and W of the final parameters (src/wdl_impl.cpp:336)
+  DBuf<double> &Ad = b.Ad, &Wd = b.Wd;
+  DBuf<T> &Ab = b.Ab, &Wt = b.Wt;
+  softmax_cols(st, Alpha.get(), Ad.get(), N, S);
+  softmax_cols(st, Lambda.get(), Wd.get(), S, M);
   if (A_out) d2h(A_out, Ad.get(), (size_t)N * S, st);
   if (W_out) d2h(W_out, Wd.get(), (size_t)S * M, st);
   if (Yhat_out) {
@@ -536,6 +608,46 @@ int rwig_wdl_batch_grad_dev(const void *K, const void *KT, long ldk, int N, int 
                             glambda, loss, checkpoint_every);
     else
       throw Error(RWIG_ERR_INVALID, "precision must be 0 (fp64) or 1 (fp32)");
+  });
+}
+
+// One training step on device buffers, the body of the reference's batch loop
+// (src/wdl_impl.cpp:316-337): A = softmax(Alpha) and W = softmax(Lambda) by
+// columns; the batched gradient of documents [first, first + D) of Y;
+// g_Lambda = g_lambda 1^T over all M documents; optimizer step `step` (1 for
+// the first, as the reference counts) on Alpha (N x S) and Lambda (S x M),
+// with their moments mA, vA (N x S) and mL, vL (S x M) updated in place
+// (zero before the first step). K, KT (leading dimension ldk) and Y (N x M)
+// in the precision given (0: double, 1: float); Alpha, Lambda, the moments
+// and loss (the batch's sum of squared residuals, may be null) FP64.
+// A fit of E epochs is the sequence of these steps over the batches; it
+// gives rwig_wdl_init's A and W bit for bit.
+int rwig_wdl_step_dev(const void *K, const void *KT, long ldk, const void *Y, int N, int M,
+                      int S, int L, int first, int D, double *Alpha, double *Lambda, double *mA,
+                      double *vA, double *mL, double *vL, int step, int optimizer, double eta,
+                      double gamma, double beta1, double beta2, double eps, double *loss,
+                      int precision, int checkpoint_every, void *stream) {
+  return guarded("rwig_wdl_step_dev", [&] {
+    RWIG_REQUIRE(K && KT && Y && Alpha && Lambda && mA && vA && mL && vL, "null argument");
+    wdl_check(N, M, S, 1.0, L, std::max(D, 1), 1, optimizer);
+    RWIG_REQUIRE(ldk >= N && ldk % 8 == 0, "ldk must be >= N and a multiple of 8");
+    RWIG_REQUIRE(D > 0 && first >= 0 && first + D <= M, "the batch must lie within the M documents");
+    RWIG_REQUIRE(step >= 1, "step counts from 1");
+    RWIG_REQUIRE(checkpoint_every >= 0, "checkpoint_every must be non-negative");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == 0) {
+      WdlStepBufs<double> b(N, S, M);
+      wdl_step<double>(st, (const double *)K, (const double *)KT, ldk, (const double *)Y, N, M,
+                       S, L, first, D, Alpha, Lambda, mA, vA, mL, vL, step, optimizer, eta,
+                       gamma, beta1, beta2, eps, loss, checkpoint_every, b);
+    } else if (precision == 1) {
+      WdlStepBufs<float> b(N, S, M);
+      wdl_step<float>(st, (const float *)K, (const float *)KT, ldk, (const float *)Y, N, M, S,
+                      L, first, D, Alpha, Lambda, mA, vA, mL, vL, step, optimizer, eta, gamma,
+                      beta1, beta2, eps, loss, checkpoint_every, b);
+    } else {
+      throw Error(RWIG_ERR_INVALID, "precision must be 0 (fp64) or 1 (fp32)");
+    }
   });
 }
 

@@ -79,6 +79,74 @@ def test_wdl_fit_matches_oracle_fp64():
         assert rel(r[k], o[k]) < 1e-9, k
 
 
+def _step_sequence(Y, C, reg, S, Alpha0, Lambda0, batch_size, epochs, max_iter, optimizer,
+                   eta, precision):
+    """The fit as a sequence of rwig_wdl_step_dev calls on device buffers;
+    returns the final Alpha, Lambda and the per-step losses."""
+    N, M = Y.shape
+    dt = torch.float64 if precision == 0 else torch.float32
+    K = np.exp(-C / reg)
+    ldk = (N + 7) // 8 * 8
+    Kt = torch.zeros((N, ldk), dtype=dt)
+    Kt[:, :N] = torch.from_numpy(K.T.copy())
+    KTt = torch.zeros((N, ldk), dtype=dt)
+    KTt[:, :N] = torch.from_numpy(K.copy())
+    Kd, KTd = Kt.cuda(), KTt.cuda()
+    Yd = torch.from_numpy(Y.T.copy()).to(dt).cuda()
+    Al = torch.from_numpy(Alpha0.T.copy()).cuda()      # column-major N x S
+    La = torch.from_numpy(Lambda0.T.copy()).cuda()     # column-major S x M
+    mom = [torch.zeros_like(Al), torch.zeros_like(Al), torch.zeros_like(La),
+           torch.zeros_like(La)]
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    batches = -(-M // batch_size)
+    losses, step = [], 0
+    for _ in range(epochs):
+        for bid in range(batches):
+            D = M % batch_size if bid == M // batch_size else batch_size
+            step += 1
+            check(lib().rwig_wdl_step_dev(Kd.data_ptr(), KTd.data_ptr(), ldk, Yd.data_ptr(), N, M,
+                                          S, max_iter, bid * batch_size, D, Al.data_ptr(),
+                                          La.data_ptr(), *[m.data_ptr() for m in mom], step,
+                                          optimizer, eta, 0.01, 0.9, 0.999, 1e-8,
+                                          loss.data_ptr(), precision, 0, st))
+            torch.cuda.synchronize()
+            losses.append(loss.item())
+    return Al.cpu().numpy().T, La.cpu().numpy().T, losses
+
+
+@pytest.mark.parametrize("precision,opt", [(0, 2), (0, 1), (1, 2)])
+def test_step_entry_point_sequence_is_the_fit(precision, opt):
+    # the fit's batch loop (src/wdl_impl.cpp:316-337) rebuilt from the device
+    # step: same A and W as rwig_wdl_init (same kernels; only the final
+    # softmax is taken on the host here), and in FP64 the oracle's fit
+    N, M, S, reg = 50, 11, 3, 0.3
+    C, Y, rng = wdl_problem(6, N, M, S)
+    Alpha0 = rng.standard_normal((N, S))
+    Lambda0 = rng.standard_normal((S, M))
+    kw = dict(batch_size=4, epochs=2, max_iter=9, optimizer=opt, eta=0.02)
+    Al, La, losses = _step_sequence(Y, C, reg, S, Alpha0, Lambda0, precision=precision, **kw)
+    A, W = O.softmax_cols(Al), O.softmax_cols(La)
+    r = rwig_amd.wdl(Y, C, reg, S, init=(Alpha0, Lambda0),
+                     precision="fp64" if precision == 0 else "fp32", **kw)
+    assert rel(A, r["A"]) < 1e-13 and rel(W, r["W"]) < 1e-13
+    assert len(losses) == 2 * 3 and all(np.isfinite(losses))
+    if precision == 0:
+        o = O.wdl(Y, C, reg, S, Alpha0, Lambda0, **kw)
+        assert rel(A, o["A"]) < 1e-9 and rel(W, o["W"]) < 1e-9
+
+
+def test_step_entry_point_refuses_bad_batches():
+    N, M, S = 20, 6, 2
+    z = torch.zeros(64, dtype=torch.float64, device="cuda")
+    p = z.data_ptr()
+    st = torch.cuda.current_stream().cuda_stream
+    for first, D, step in ((4, 3, 1), (0, 0, 1), (-1, 2, 1), (0, 2, 0)):
+        rc = lib().rwig_wdl_step_dev(p, p, 24, p, N, M, S, 5, first, D, p, p, p, p, p, p, step,
+                                     2, 0.01, 0.01, 0.9, 0.999, 1e-8, None, 0, 0, st)
+        assert rc != 0, (first, D, step)
+
+
 @pytest.mark.parametrize("opt", [0, 1])
 def test_wdl_fit_other_optimizers(opt):
     N, M, S = 40, 8, 2

@@ -163,10 +163,17 @@ k_gemv_t(const double *__restrict__ K, int M, int N, long ldk,
     for (int t = 0; t < SM; ++t)
       if (t < S) acc[t] += kij * U[i + (long)t * ldu];
   }
+  // the op indexes the dots at run time: staged in LDS, so that acc stays in
+  // registers (a private array indexed at run time would live in scratch)
+  __shared__ double dots[kWavesPerBlock][SM];
+  double *dw = dots[threadIdx.x / kWave];
 #pragma unroll
   for (int t = 0; t < SM; ++t)
-    if (t < S) acc[t] = wave_sum(acc[t]);
-  if (lane == 0) op(j, acc);
+    if (t < S) {
+      const double v = wave_sum(acc[t]);
+      if (lane == 0) dw[t] = v;
+    }
+  if (lane == 0) op(j, (const double *)dw);
 }
 
 #define RWIG_DISPATCH_S(S, SM, ...)                                            \
@@ -304,16 +311,26 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
       g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] * sc : T(0);
     }
   for (int i = threadIdx.x; i < M; i += kTB) {
-    T f[SM];
-#pragma unroll
-    for (int s = 0; s < SM; ++s) f[s] = (s < S) ? F[i + (long)s * ldf] * sc : T(0);
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-      if (j0 + c >= N) continue;
-      const T x = C[i + (long)(j0 + c) * ldc] * sc;
+    if constexpr (CB == 1) {
+      // one column: each F value is used once, so it is loaded where it is
+      // used rather than held for all slices (no scratch at S = 32 in FP64)
+      if (j0 >= N) continue;
+      const T x = C[i + (long)j0 * ldc] * sc;
 #pragma unroll
       for (int s = 0; s < SM; ++s)
-        if (s < S) st[c][s].add((x - f[s]) - g[c][s]);
+        if (s < S) st[0][s].add((x - F[i + (long)s * ldf] * sc) - g[0][s]);
+    } else {
+      T f[SM];
+#pragma unroll
+      for (int s = 0; s < SM; ++s) f[s] = (s < S) ? F[i + (long)s * ldf] * sc : T(0);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        if (j0 + c >= N) continue;
+        const T x = C[i + (long)(j0 + c) * ldc] * sc;
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+          if (s < S) st[c][s].add((x - f[s]) - g[c][s]);
+      }
     }
   }
   // wavefront, then workgroup merge
@@ -335,7 +352,10 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
   if (threadIdx.x < CB) {
     const int c = threadIdx.x;
     if (j0 + c >= N) return;
-    T res[SM];
+    // the results go back into this column's own LDS entries (read by this
+    // thread only), which the op indexes at run time: a private array would
+    // live in scratch
+    T *res = &shm[0][c * SM];
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
       if (s >= S) continue;
@@ -345,7 +365,7 @@ k_lse_col(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__
       for (int v = 1; v < kWavesPerBlock; ++v) a.merge(shm[v][c * SM + s], sht[v][c * SM + s]);
       res[s] = a.value(back);
     }
-    op(j0 + c, res);
+    op(j0 + c, (const T *)res);
   }
 }
 
@@ -586,19 +606,32 @@ k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
       g[c][s] = (s < S && j0 + c < N) ? G[(j0 + c) + (long)s * ldg] * sc : T(0);
     }
   for (int i = threadIdx.x; i < M; i += kTB) {
-    T fs[SM], xv[SM];  // fs = (F + rshift) * sc: the row's two shifts in one
-#pragma unroll
-    for (int s = 0; s < SM; ++s) {
-      fs[s] = (s < S) ? (F[i + (long)s * ldf] + rshift[i + (long)s * M]) * sc : T(0);
-      xv[s] = (s < S) ? x[i + (long)s * M] : T(0);
-    }
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-      if (j0 + c >= N) continue;
-      const T cij = C[i + (long)(j0 + c) * ldc] * sc;
+    if constexpr (CB == 1) {
+      // one column: the row's values are loaded where they are used (no
+      // scratch at S = 32 in FP64)
+      if (j0 >= N) continue;
+      const T cij = C[i + (long)j0 * ldc] * sc;
 #pragma unroll
       for (int s = 0; s < SM; ++s)
-        if (s < S) acc[c][s] += sm_exp2(-((cij - fs[s]) - g[c][s])) * xv[s];
+        if (s < S) {
+          const T fs = (F[i + (long)s * ldf] + rshift[i + (long)s * M]) * sc;
+          acc[0][s] += sm_exp2(-((cij - fs) - g[0][s])) * x[i + (long)s * M];
+        }
+    } else {
+      T fs[SM], xv[SM];  // fs = (F + rshift) * sc: the row's two shifts in one
+#pragma unroll
+      for (int s = 0; s < SM; ++s) {
+        fs[s] = (s < S) ? (F[i + (long)s * ldf] + rshift[i + (long)s * M]) * sc : T(0);
+        xv[s] = (s < S) ? x[i + (long)s * M] : T(0);
+      }
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        if (j0 + c >= N) continue;
+        const T cij = C[i + (long)(j0 + c) * ldc] * sc;
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+          if (s < S) acc[c][s] += sm_exp2(-((cij - fs[s]) - g[c][s])) * xv[s];
+      }
     }
   }
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -616,7 +649,9 @@ k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
   if (threadIdx.x < CB) {
     const int c = threadIdx.x;
     if (j0 + c >= N) return;
-    T res[SM];
+    // results into this column's own LDS entries (read by this thread only),
+    // which the op indexes at run time
+    T *res = &sha[0][c * SM];
 #pragma unroll
     for (int s = 0; s < SM; ++s) {
       if (s >= S) continue;
@@ -624,7 +659,7 @@ k_wcol(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict__ F,
       for (int u = 1; u < kWavesPerBlock; ++u) v += sha[u][c * SM + s];
       res[s] = v;
     }
-    op(j0 + c, res);
+    op(j0 + c, (const T *)res);
   }
 }
 
@@ -789,10 +824,13 @@ k_cols_apply(const T *__restrict__ v, int N, int S, Op op, const LoopState *ls) 
   if (ls && !ls->active) return;
   const int j = blockIdx.x * kWave + threadIdx.x;
   if (j >= N) return;
-  T res[SM];
+  // the op indexes the values at run time: staged in LDS rather than in a
+  // private array (which would live in scratch at SM = 32)
+  __shared__ T res[kWave][SM];
+  T *r = res[threadIdx.x];
 #pragma unroll
-  for (int s = 0; s < SM; ++s) res[s] = s < S ? v[(long)s * N + j] : T(0);
-  op(j, res);
+  for (int s = 0; s < SM; ++s) r[s] = s < S ? v[(long)s * N + j] : T(0);
+  op(j, (const T *)r);
 }
 
 template <typename T, int SM, class Op>

@@ -179,7 +179,11 @@ def test_barycenter_log_matches_oracle(seed, M, N, S, reg, wg):
     assert r["iter"] == o["iter"]
     keys = ["b", "F", "G"] + (["grad_A", "grad_w", "loss"] if wg else [])
     for k in keys:
-        assert rel(r[k], o[k]) < TOL_FP64, k
+        # the potentials are differences of terms of size reg * log(.): an
+        # array of them that is zero up to rounding (G at S = 1) is compared
+        # on the scale reg, not 1e-6
+        floor = reg if k in ("F", "G") else 1e-6
+        assert rel(r[k], o[k], floor=floor) < TOL_FP64, k
 
 
 def test_c1_vanilla_256_config():

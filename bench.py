@@ -506,6 +506,25 @@ def c5_leg(log, paced_s=8.0, graphs=True):
                    peak_hbm_bytes=st["peak_bytes"], all_converged=ok)
         log(f"C5 paced at {rate:.0f}/s: {st['solves_per_s']:.0f} solves/s, "
             f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, all converged {ok}")
+        # unloaded latency: requests of the same mix, one at a time (each
+        # submitted after the previous result), submit to completion
+        reqs = list(W.c5_requests(sizes, 48, seed=2008))
+        lat, by_size = [], {"n<=512": [], "512<n<=2048": [], "n>2048": []}
+        for q in reqs:
+            r = s.result(submit(s, ids, q))
+            n = sizes[q["cloud"]]
+            key = "n<=512" if n <= 512 else ("512<n<=2048" if n <= 2048 else "n>2048")
+            lat.append(r["latency_ms"])
+            by_size[key].append(r["latency_ms"])
+        lat = np.array(lat)
+        out["unloaded"] = {
+            "requests": len(reqs), "p50_ms": float(np.percentile(lat, 50)),
+            "p90_ms": float(np.percentile(lat, 90)), "max_ms": float(lat.max()),
+            "p50_ms_by_size": {k: (float(np.median(v)) if v else None)
+                               for k, v in by_size.items()}}
+        log(f"C5 unloaded (one request at a time, {len(reqs)} requests): p50 "
+            f"{out['unloaded']['p50_ms']:.1f} ms, p90 {out['unloaded']['p90_ms']:.1f} ms, max "
+            f"{out['unloaded']['max_ms']:.1f} ms; p50 by size {out['unloaded']['p50_ms_by_size']}")
     return out
 
 

@@ -201,8 +201,8 @@ void dense_kernel_build(hipStream_t st, const double *C, long ldc, int M, int Nc
                         DenseKernel &dk) {
   dk.M = M;
   dk.Nc = Nc;
-  dk.ldm = pad8(M);
-  dk.ldn = pad8(Nc);
+  dk.ldm = pad64(M);
+  dk.ldn = pad64(Nc);
   dk.Kb.reset((size_t)dk.ldm * Nc);
   dk.KbT.reset((size_t)dk.ldn * M);
   dk.Kd.reset((size_t)dk.ldm * Nc);

@@ -16,7 +16,10 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 
 struct DenseKernel {
   int M = 0, Nc = 0;
-  long ldm = 0, ldn = 0;  // padded M, padded Nc (multiples of 8)
+  // padded M, padded Nc (multiples of 64, zero past M and Nc): a BF16 product
+  // may run its contraction over the padded length in whole 64-element
+  // k-steps (direct-to-LDS tiles) with the same result as over M or Nc
+  long ldm = 0, ldn = 0;
   DBuf<uint16_t> Kb, KbT;  // Kb: Kb[i + j*ldm] = K(i,j); KbT: KbT[j + i*ldn] = K(i,j)
   DBuf<double> Kd, KdT;
   // KCT[j + i*ldn] = K(i,j) C(i,j): the transport cost of the batched solver as
@@ -26,10 +29,11 @@ struct DenseKernel {
 };
 
 inline long pad8(long n) { return (n + 7) / 8 * 8; }
+inline long pad64(long n) { return (n + 63) / 64 * 64; }
 
 // Device bytes of a DenseKernel of an M x Nc cost.
 inline size_t dense_kernel_bytes(int M, int Nc) {
-  return (size_t)pad8(M) * Nc * (2 + 8) + (size_t)pad8(Nc) * M * (2 + 8);
+  return (size_t)pad64(M) * Nc * (2 + 8) + (size_t)pad64(Nc) * M * (2 + 8);
 }
 
 // Fills dk from the device cost C (M x Nc, leading dimension ldc).

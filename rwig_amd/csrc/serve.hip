@@ -329,6 +329,9 @@ struct Group {
     drop_graphs();  // they hold the old buffers' addresses
     for (Slab *s : slabs()) {
       DBuf<char> nb(s->per_col * (size_t)ncap);
+      // zero first: the rows of W and X past M and N must read as zeros (the
+      // BF16 products contract over the padded lengths)
+      RWIG_HIP_CHECK(hipMemsetAsync(nb.get(), 0, s->per_col * (size_t)ncap, stream->get()));
       if (cap) d2d(nb.get(), s->buf.get(), s->per_col * (size_t)cap, stream->get());
       stream->sync();
       s->buf = std::move(nb);
@@ -585,13 +588,13 @@ struct Group {
     with_tiles<uint16_t>(tile_cols(), [&](auto cfg) {
       using CF = decltype(cfg);
       gemm_tn<uint16_t, CF>(
-          st, Mi, cp, Ni, dk.KbT.get(), ldni, Xp, ldni,
+          st, Mi, cp, (int)ldni, dk.KbT.get(), ldni, Xp, ldni,
           [=] __device__(int i, int t, float y) {
             if (!dn[t]) Ycp[i + (long)t * Mi] = y;
           },
           act, cnt + kNact, skb(ks_y()));
       gemm_tn<uint16_t, CF>(
-          st, Ni, cp, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+          st, Ni, cp, (int)ldmi, dk.Kb.get(), ldmi, Wp, ldmi,
           [=] __device__(int j, int t, float z) {
             const long q = j + (long)t * Ni;
             if (!dn[t]) sbp[q] = (float)(c0p[q] / (double)z);
@@ -639,7 +642,7 @@ struct Group {
     uint16_t *Wp = W.p<uint16_t>(), *Xp = X.p<uint16_t>();
     const DenseKernel &dk = gk->dk;
     product<uint16_t>(
-        st, Ni, Mi, dk.Kb.get(), ldmi, Wp, ldmi,
+        st, Ni, (int)ldmi, dk.Kb.get(), ldmi, Wp, ldmi,
         [=] __device__(int j, int t, float z) {
           if (s_it >= lim[t]) return;
           const long q = j + (long)t * Ni;
@@ -684,7 +687,7 @@ struct Group {
       }
     });
     product<uint16_t>(
-        st, Mi, Ni, dk.KbT.get(), ldni, Xp, ldni,
+        st, Mi, (int)ldni, dk.KbT.get(), ldni, Xp, ldni,
         [=] __device__(int i, int t, float y) {
           if (s_it >= lim[t]) return;
           const long q = i + (long)t * Mi;
@@ -948,8 +951,8 @@ class ServeEngine {
     Group probe;
     probe.M = M;
     probe.N = N;
-    probe.ldm = pad8(M);
-    probe.ldn = pad8(N);
+    probe.ldm = pad64(M);
+    probe.ldn = pad64(N);
     probe.layout();
     const int cols = std::max(kColPage, S);
     return dense_kernel_bytes(M, N) + probe.bytes(cols);
@@ -1021,8 +1024,8 @@ class ServeEngine {
     Group probe;
     probe.M = c.M;
     probe.N = c.N;
-    probe.ldm = pad8(c.M);
-    probe.ldn = pad8(c.N);
+    probe.ldm = pad64(c.M);
+    probe.ldn = pad64(c.N);
     probe.layout();
     if (g) {
       int ncap = g->cap;
@@ -1045,8 +1048,8 @@ class ServeEngine {
     ng->reg = r->reg;
     ng->M = c.M;
     ng->N = c.N;
-    ng->ldm = pad8(c.M);
-    ng->ldn = pad8(c.N);
+    ng->ldm = pad64(c.M);
+    ng->ldn = pad64(c.N);
     ng->layout();
     ng->graphs = graphs_;
     ng->absorb_every = absorb_every_;
@@ -1301,8 +1304,8 @@ int rwig_serve_footprint(int M, int N, int columns, double *kernel_bytes, double
     Group probe;
     probe.M = M;
     probe.N = N;
-    probe.ldm = pad8(M);
-    probe.ldn = pad8(N);
+    probe.ldm = pad64(M);
+    probe.ldn = pad64(N);
     probe.layout();
     if (kernel_bytes) *kernel_bytes = (double)dense_kernel_bytes(M, N);
     if (group_bytes) *group_bytes = (double)probe.bytes(columns);

@@ -277,8 +277,8 @@ struct Group {
                     tiles_work_elems<uint16_t>(M, c, ks_y()));
   }
   size_t wk_d_elems(int c) const {
-    return std::max(splitk_work_elems<double>(M, c, ks_r()),
-                    splitk_work_elems<double>(N, c, ks_c()));
+    return std::max(tiles_work_elems<double>(M, c, ks_r()),
+                    tiles_work_elems<double>(N, c, ks_c()));
   }
 
   // Without a k split (small n) the product kernel runs the update itself:
@@ -561,16 +561,22 @@ struct Group {
       bep[q] = (float)bq;
       Bdp[j + t * ldni] = bq;
     });
-    // (the FP64 products keep the default tiles: with their k split and the
-    // register path that ragged n takes, smaller FP64 tiles measured slower)
-    gemm_tn<double>(
-        st, Mi, cp, Ni, dk.KdT.get(), ldni, Bdp, ldni,
-        [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
-        act, cnt + kNact, skd(ks_r()));
-    gemm_tn<double>(
-        st, Ni, cp, Mi, dk.Kd.get(), ldmi, Adp, ldmi,
-        [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
-        act, cnt + kNact, skd(ks_c()));
+    // the FP64 products contract over the zero-padded lengths too (whole
+    // k-steps: the direct-to-LDS tiles) and take their tiles from the column
+    // capacity, up to the 64 x 64 tiles (the 128 x 128 ones, with the k
+    // split fixed by the rows, measured slower on wide groups: 1,024 columns
+    // of n = 2048 in 8 splits); the k split does not depend on the tiles
+    with_tiles<double>(std::min(tile_cols(), 768), [&](auto cfg) {
+      using CF = decltype(cfg);
+      gemm_tn<double, CF>(
+          st, Mi, cp, (int)ldni, dk.KdT.get(), ldni, Bdp, ldni,
+          [=] __device__(int i, int t, double v) { r0p[i + (long)t * Mi] = v * Adp[i + (long)t * ldmi]; },
+          act, cnt + kNact, skd(ks_r()));
+      gemm_tn<double, CF>(
+          st, Ni, cp, (int)ldmi, dk.Kd.get(), ldmi, Adp, ldmi,
+          [=] __device__(int j, int t, double v) { c0p[j + (long)t * Ni] = v * Bdp[j + (long)t * ldni]; },
+          act, cnt + kNact, skd(ks_c()));
+    });
     hipLaunchKernelGGL(k_serve_err, dim3(cp, err_blocks()), dim3(256), 0, st, Mi, Ni, rg, d_areq(),
                        d_riter(), kind.p<int>(), ns.p<int>(), wcol.p<double>(), maxit.p<int>(),
                        tol.p<double>(), ap, r0p, bp, c0p, Gp, switch_thr, dn, md,

@@ -444,8 +444,9 @@ struct Group {
   }
 
   // Column count for the tile policy: at most 32 columns take the 32 x 32
-  // tiles here, not the 16 x 32 ones. Those pay off on the direct-to-LDS path
-  // only; ragged n takes the register path, where they measured 7% slower.
+  // tiles here, not the 16 x 32 ones (on the register path they measured 7%
+  // slower; with the padded contraction, on the direct-to-LDS path, no
+  // faster).
   int tile_cols() const { return std::max(cap, 33); }
 
   // A product over the running columns (act, count from the device lists), in

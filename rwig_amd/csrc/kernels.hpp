@@ -273,8 +273,18 @@ template <typename T, typename X = T> struct SoftMin {
     const T mn = fmin(m, mg);
     if (m != mn) t *= ex2<X>(mn - m);
     m = mn;
+    if constexpr (sizeof(X) < sizeof(T)) {
+      // mixed form: the group's G terms (each <= 1) are summed in X and
+      // converted once, which saves a conversion per term; the group sum's
+      // rounding (a few X ulps) is of the order of the X exponentials' own
+      X tg = X(0);
 #pragma unroll
-    for (int u = 0; u < G; ++u) t += ex2<X>(m - r[u]);
+      for (int u = 0; u < G; ++u) tg += sm_exp2((X)(m - r[u]));
+      t += (T)tg;
+    } else {
+#pragma unroll
+      for (int u = 0; u < G; ++u) t += ex2<X>(m - r[u]);
+    }
   }
   // the soft-min in the caller's units: (m - log2 t) / sc
   __device__ T value(T back) const { return (m - log2(t)) * back; }

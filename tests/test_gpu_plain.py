@@ -153,6 +153,7 @@ def bary_problem(seed, M, N, S):
     (31, 50, 40, 3, 0.1, True),
     (32, 256, 256, 8, 0.05, True),
     (33, 100, 120, 5, 0.1, False),
+    (34, 60, 64, 32, 0.1, True),   # 32 measures: the widest slice kernels
 ])
 def test_barycenter_parallel_matches_oracle(seed, M, N, S, reg, wg):
     A, C, w, bext = bary_problem(seed, M, N, S)
@@ -170,6 +171,8 @@ def test_barycenter_parallel_matches_oracle(seed, M, N, S, reg, wg):
     (41, 40, 50, 3, 0.05, True),
     (42, 200, 150, 4, 0.02, True),
     (43, 64, 64, 1, 0.1, False),
+    (44, 60, 64, 32, 0.1, True),   # 32 measures: one-column blocks at S = 32
+    (45, 70, 50, 16, 0.05, True),
 ])
 def test_barycenter_log_matches_oracle(seed, M, N, S, reg, wg):
     A, C, w, bext = bary_problem(seed, M, N, S)

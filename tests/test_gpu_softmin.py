@@ -39,7 +39,8 @@ def reference(C, F, G, reg):
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
 @pytest.mark.parametrize("M,N,S,reg", [(100, 70, 1, 0.05), (300, 257, 3, 0.01),
-                                       (1000, 600, 8, 0.004), (64, 2000, 5, 0.1)])
+                                       (1000, 600, 8, 0.004), (64, 2000, 5, 0.1),
+                                       (200, 150, 16, 0.02), (130, 90, 32, 0.05)])
 def test_softmin_passes_match_reference(M, N, S, reg, prec, tol):
     C, F, G = problem(M + N + S, M, N, S)
     rows_ref, cols_ref, scale = reference(C, F, G, reg)

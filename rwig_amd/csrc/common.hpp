@@ -179,6 +179,16 @@ __device__ __forceinline__ float wave_min(float v) {
   for (int o = kWave / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
   return v;
 }
+// Index of the calling wavefront in a one-dimensional workgroup, as a value
+// the compiler knows to be uniform across the wavefront (every lane of a
+// wavefront has the same threadIdx.x / 64). Loop bounds and column offsets
+// derived from it then stay in scalar registers: a column offset of a
+// global load becomes one scalar multiply instead of six vector
+// instructions per load, two of them quarter-rate multiplies.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);

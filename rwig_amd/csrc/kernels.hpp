@@ -468,7 +468,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
                T *__restrict__ pm, T *__restrict__ ps, const LoopState *ls) {
   if (ls && !ls->active) return;
   __shared__ T shm[kWavesPerBlock][kRowsPerBlock], sht[kWavesPerBlock][kRowsPerBlock];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_id();
   const int i = blockIdx.x * kRowsPerBlock + lane;
   const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
@@ -490,7 +490,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
       for (; jj + (GR - 1) * kWavesPerBlock < nc; jj += GR * kWavesPerBlock) {
         T x[GR];
 #pragma unroll
-        for (int u = 0; u < GR; ++u) x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+        for (int u = 0; u < GR; ++u) x[u] = (C + (long)(t0 + jj + u * kWavesPerBlock) * ldc)[i];
         if constexpr (RowGroup<T, SM, X>::two_step) {
 #pragma unroll
           for (int s = 0; s < SM; ++s) {
@@ -515,7 +515,7 @@ k_lse_row_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restr
       }
       // the last columns of the tile, one at a time
       for (; jj < nc; jj += kWavesPerBlock) {
-        const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
+        const T xs = (C + (long)(t0 + jj) * ldc)[i] * sc;
         const T *g = gs + jj * SM;
 #pragma unroll
         for (int s = 0; s < SM; ++s) st[s].add(xs - g[s]);
@@ -702,7 +702,7 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
             const LoopState *ls) {
   if (ls && !ls->active) return;
   __shared__ T sha[kWavesPerBlock][kRowsPerBlock];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_id();
   const int i = blockIdx.x * kRowsPerBlock + lane;
   const int c = blockIdx.y;
   const int j0 = c * cw, j1 = min(N, j0 + cw);
@@ -733,7 +733,7 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
         T x[kColUnroll];
 #pragma unroll
         for (int u = 0; u < kColUnroll; ++u)
-          x[u] = C[i + (long)(t0 + jj + u * kWavesPerBlock) * ldc];
+          x[u] = (C + (long)(t0 + jj + u * kWavesPerBlock) * ldc)[i];
 #pragma unroll
         for (int u = 0; u < kColUnroll; ++u) {
           const int q = (jj + u * kWavesPerBlock) * SM;
@@ -744,7 +744,7 @@ k_wrow_part(const T *__restrict__ C, int M, int N, long ldc, const T *__restrict
         }
       }
       for (; jj < nc; jj += kWavesPerBlock) {
-        const T xs = C[i + (long)(t0 + jj) * ldc] * sc;
+        const T xs = (C + (long)(t0 + jj) * ldc)[i] * sc;
         const T *g = gh + jj * SM, *yv = ys + jj * SM;
 #pragma unroll
         for (int s = 0; s < SM; ++s) acc[s] += ex2_wide<X>(-((xs - f[s]) - g[s])) * yv[s];

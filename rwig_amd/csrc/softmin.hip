@@ -5,7 +5,6 @@
 // of the chunk partials, except in rwig_softmin_rows_noalloc_dev, whose
 // partials live in the file-scope device array below.
 #include <mutex>
-#include <unordered_map>
 
 #include "kernels.hpp"
 
@@ -22,20 +21,22 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 namespace {
 
 // Device address of `scratch` on the current device, looked up once per
-// device with that device current. The array's name is a host-side stand-in
-// in host code: kernels must be given this address instead.
+// device with that device current (a fixed table: no host allocation either).
+// The array's name is a host-side stand-in in host code: kernels must be
+// given this address instead.
+constexpr int kMaxDevices = 64;
 void *scratch_address() {
   static std::mutex mu;
-  static std::unordered_map<int, void *> addr;
+  static void *addr[kMaxDevices] = {};
   int dev = 0;
   RWIG_HIP_CHECK(hipGetDevice(&dev));
+  RWIG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
   std::lock_guard<std::mutex> lk(mu);
-  auto it = addr.find(dev);
-  if (it != addr.end()) return it->second;
+  if (addr[dev]) return addr[dev];
   void *p = nullptr;
   RWIG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
   RWIG_REQUIRE(p != nullptr, "no device address for the soft-min scratch array");
-  addr.emplace(dev, p);
+  addr[dev] = p;
   return p;
 }
 

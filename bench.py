@@ -211,6 +211,49 @@ def gemm_only():
           "5 FP64 products 4096 x 1024 x 4096", flush=True)
 
 
+def c3_resident(A, C, w, bext, eps, L, log, reps=3):
+    """C3 with its inputs resident in HBM (rwig_barycenter_log_ck_dev): C in the
+    working precision's type, outputs left on the device; ms per forward +
+    adjoint, timed on the host around calls that end with a stream
+    synchronisation."""
+    import torch
+
+    from rwig_amd._lib import check, lib
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    M, S = A.shape
+    N = C.shape[1]
+    Cf = np.asfortranarray(C).ravel(order="F")
+    C64 = torch.from_numpy(Cf).to(dev)
+    C32 = C64.to(torch.float32)
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).to(dev)
+    wd = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
+    bd = torch.from_numpy(np.ascontiguousarray(bext)).to(dev)
+    o = {k: torch.empty(n, dtype=torch.float64, device=dev)
+         for k, n in [("F", M * S), ("G", N * S), ("b", N), ("gA", M * S), ("gw", S),
+                      ("loss", 1)]}
+    it, er, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    res = {}
+    for name, prec in (("fp32", 1), ("fp32exp", 2), ("fp64", 0)):
+        Cd = C32 if prec == 1 else C64
+
+        def call():
+            check(lib().rwig_barycenter_log_ck_dev(
+                o["F"].data_ptr(), o["G"].data_ptr(), o["b"].data_ptr(), o["gA"].data_ptr(),
+                o["gw"].data_ptr(), o["loss"].data_ptr(), ctypes.byref(it), ctypes.byref(er),
+                Ad.data_ptr(), wd.data_ptr(), Cd.data_ptr(), bd.data_ptr(), M, N, S, eps, 1, L,
+                0.0, 20, 1, prec, ctypes.byref(rs), st))
+        call()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            call()
+        res[f"{name}_ms_per_gradient_resident"] = (time.perf_counter() - t) / reps * 1e3
+        log(f"C3 {name}, inputs resident: {res[f'{name}_ms_per_gradient_resident']:.1f} ms "
+            "per forward + adjoint")
+    return res
+
+
 def c3_leg(log, cpu=False):
     """Barycenter with unrolled gradient (64 x 64 grid, S = 8, eps 0.004,
     L = 200, checkpoints every 20): one forward + adjoint in FP32, in the
@@ -236,6 +279,7 @@ def c3_leg(log, cpu=False):
         out[f"{prec}_ms_per_gradient"] = (time.perf_counter() - t) / reps * 1e3
         log(f"C3 {prec}: {out[f'{prec}_ms_per_gradient']:.1f} ms per forward + adjoint "
             "(host FP64 inputs and outputs included)")
+    out.update(c3_resident(A, C, w, bext, eps, L, log))
     if cpu:
         from oracle import oracle as O
         th = host_info()["threads"]

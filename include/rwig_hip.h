@@ -110,6 +110,23 @@ int rwig_barycenter_log_ck(double *F, double *G, double *b, double *grad_A,
                            double zero_tol, int checkpoint_every,
                            int seed_mode, int precision, int *return_status);
 
+/* rwig_barycenter_log_ck on DEVICE buffers, enqueued on `stream`: C (M x N,
+ * column-major, leading dimension M) in the working precision's type (FP32
+ * for precision 1, FP64 for precisions 0 and 2), A (M x S), w (S) and b_ext
+ * (N, with withgrad) in FP64; F (M x S), G (N x S), b (N), grad_A (M x S),
+ * grad_w (S) and loss (1) are FP64 device outputs, each may be null (not
+ * returned). iter_out, err_out and return_status are host values; the call
+ * synchronises the stream before returning. For inputs already resident in
+ * HBM (no counterpart in the reference, whose entry points take R vectors). */
+int rwig_barycenter_log_ck_dev(double *F, double *G, double *b, double *grad_A,
+                               double *grad_w, double *loss, int *iter_out,
+                               double *err_out, const double *A, const double *w,
+                               const void *C, const double *b_ext, int M, int N,
+                               int S, double reg, int withgrad, int max_iter,
+                               double zero_tol, int checkpoint_every,
+                               int seed_mode, int precision, int *return_status,
+                               void *stream);
+
 /* D log barycenters without gradient that share A (M x S) and C (M x N),
  * document d weighted by W(:, d) (S x D): B (N x D) receives each document's
  * normalised barycenter as barycenter_log_cpp computes it

@@ -33,6 +33,8 @@ SIGNATURES = {
                              _i, _i, _d, _i, _i, _d, _ip], _i),
     "rwig_barycenter_log_ck": ([_dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
                                 _i, _i, _i, _d, _i, _i, _d, _i, _i, _i, _ip], _i),
+    "rwig_barycenter_log_ck_dev": ([_vp, _vp, _vp, _vp, _vp, _vp, _ip, _dp, _vp, _vp, _vp, _vp,
+                                    _i, _i, _i, _d, _i, _i, _d, _i, _i, _i, _ip, _vp], _i),
     "rwig_barycenter_log_batched": ([_dp, _ip, _dp, _dp, _dp, _dp, _i, _i, _i, _i, _d, _i,
                                      _d], _i),
     "rwig_sinkhorn_batched_dev": ([_vp, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp, _i, _d, _i,
@@ -113,6 +115,17 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m rwig_amd.build`")
+        # The PyTorch ROCm wheel ships its own HIP and HSA runtimes. The HSA
+        # runtime has the same soname as the one this library links, so the
+        # process keeps whichever is loaded first. Torch's HIP does not run on
+        # the system's newer HSA runtime (torch then finds no GPU), while this
+        # library runs on torch's. Where torch is installed it is therefore
+        # imported before the library is loaded, whatever the caller's import
+        # order; without torch nothing changes.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(handle, name)

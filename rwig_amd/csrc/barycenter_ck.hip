@@ -35,6 +35,8 @@
 // the reference's ||b - b_ext||^2).
 #include <algorithm>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace rwig {
@@ -518,11 +520,92 @@ BaryCkResult run_ck(hipStream_t st, double *Fh, double *Gh, double *bh, double *
   return res;
 }
 
+// The same solve on device buffers: C in T (column-major, leading dimension
+// M), A, w and b_ext in FP64; F, G and b are written in FP64 (converted from
+// T), the gradient and loss as the solve leaves them. Null F, G or b: not
+// returned.
+template <typename T, typename X>
+BaryCkResult run_ck_dev(hipStream_t st, double *F, double *G, double *b, double *gA, double *gw,
+                        double *loss, const double *A, const double *w, const T *C,
+                        const double *bext, int M, int N, int S, double reg, bool wg, int maxiter,
+                        double tol, int ck, int seed) {
+  const long MS = (long)M * S, NS = (long)N * S;
+  Arena ar;
+  T *dw = ar.make<T>(S);
+  launch_map(st, (long)S, [=] __device__(long k) { dw[k] = (T)w[k]; });
+  T *dbext = nullptr;
+  if (wg) {
+    dbext = ar.make<T>(N);
+    launch_map(st, (long)N, [=] __device__(long k) { dbext[k] = (T)bext[k]; });
+  }
+  constexpr bool same = std::is_same<T, double>::value;
+  auto out = [&](double *h, long n) -> T * {
+    if constexpr (same) {
+      if (h) return h;
+    }
+    return ar.make<T>(n);
+  };
+  T *db = out(b, N), *dF = out(F, MS), *dG = out(G, NS);
+  double *dgA = wg && gA ? gA : ar.make<double>(MS);
+  double *dgw = wg && gw ? gw : ar.make<double>(S);
+  double *dloss = wg && loss ? loss : ar.make<double>(1);
+  const BaryCkResult res = barycenter_log_ck<T, X>(st, A, C, dw, dbext, M, N, S, reg, wg, maxiter,
+                                                tol, ck, seed, db, dF, dG, dgA, dgw, dloss);
+  if constexpr (!same) {
+    auto widen = [&](double *h, const T *d, long n) {
+      if (h) launch_map(st, n, [=] __device__(long k) { h[k] = (double)d[k]; });
+    };
+    widen(b, db, N);
+    widen(F, dF, MS);
+    widen(G, dG, NS);
+  }
+  RWIG_HIP_CHECK(hipStreamSynchronize(st));
+  return res;
+}
+
 }  // namespace
 
 }  // namespace rwig
 
 using namespace rwig;
+
+extern "C" int rwig_barycenter_log_ck_dev(double *F, double *G, double *b, double *grad_A,
+                                          double *grad_w, double *loss, int *iter_out,
+                                          double *err_out, const double *A, const double *w,
+                                          const void *C, const double *b_ext, int M, int N,
+                                          int S, double reg, int withgrad, int max_iter,
+                                          double zero_tol, int checkpoint_every, int seed_mode,
+                                          int precision, int *return_status_out, void *stream) {
+  return guarded("rwig_barycenter_log_ck_dev", [&] {
+    RWIG_REQUIRE(M > 0 && N > 0, "matrix dimensions must be positive");
+    RWIG_REQUIRE(max_iter >= 0, "max_iter must be non-negative");
+    RWIG_REQUIRE(reg > 0.0, "reg must be positive");
+    RWIG_REQUIRE(S > 0 && S <= 32, "S must be in [1, 32]");
+    RWIG_REQUIRE(A && w && C, "A, w and C are required");
+    RWIG_REQUIRE(!withgrad || b_ext, "b_ext is required with withgrad");
+    RWIG_REQUIRE(checkpoint_every >= 1, "checkpoint_every must be positive");
+    RWIG_REQUIRE(seed_mode >= 0 && seed_mode <= 2, "seed_mode must be 0, 1 or 2");
+    RWIG_REQUIRE(precision >= 0 && precision <= 2,
+                 "precision must be 0 (FP64), 1 (FP32) or 2 (FP64 arguments, FP32 exponentials)");
+    hipStream_t st = (hipStream_t)stream;
+    const bool wg = withgrad != 0;
+    const BaryCkResult res =
+        precision == 0
+            ? run_ck_dev<double, double>(st, F, G, b, grad_A, grad_w, loss, A, w,
+                                         (const double *)C, b_ext, M, N, S, reg, wg, max_iter,
+                                         zero_tol, checkpoint_every, seed_mode)
+        : precision == 1
+            ? run_ck_dev<float, float>(st, F, G, b, grad_A, grad_w, loss, A, w, (const float *)C,
+                                       b_ext, M, N, S, reg, wg, max_iter, zero_tol,
+                                       checkpoint_every, seed_mode)
+            : run_ck_dev<double, float>(st, F, G, b, grad_A, grad_w, loss, A, w,
+                                        (const double *)C, b_ext, M, N, S, reg, wg, max_iter,
+                                        zero_tol, checkpoint_every, seed_mode);
+    if (iter_out) *iter_out = res.iter;
+    if (err_out) *err_out = res.err;
+    if (return_status_out) *return_status_out = status_of(res.err, zero_tol, res.iter, max_iter);
+  });
+}
 
 extern "C" int rwig_barycenter_log_batched(double *B, int *iters_out, double *errs_out,
                                            const double *A, const double *W, const double *C,

@@ -224,3 +224,55 @@ def test_c3_fp32_against_fp64():
           f"loss {el:.2e}")
     assert eb < TOL_FP32_B and el < TOL_FP32_B
     assert egA < TOL_FP32_GRAD and egw < TOL_FP32_GRAD
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_device_entry_point_equals_the_host_one(precision):
+    # rwig_barycenter_log_ck_dev runs the same solve on device buffers (C in
+    # the working precision's type): every output equals the host entry
+    # point's bit for bit, gradient included, and null outputs are allowed
+    import ctypes
+
+    import torch
+
+    from rwig_amd._lib import check, lib
+
+    A, C, w, bext = bary_problem(77, 300, 260, 3)
+    reg, L = 0.02, 50
+    prec_name = {0: "fp64", 1: "fp32", 2: "fp32exp"}[precision]
+    r = ck(A, C, w, bext, reg, L, with_grad=True, seed="unnormalised", checkpoint_every=7,
+           precision=prec_name)
+    M, S = A.shape
+    N = C.shape[1]
+    dev = torch.device("cuda:0")
+    cdt = torch.float32 if precision == 1 else torch.float64
+    Cd = torch.from_numpy(np.asfortranarray(C).ravel(order="F")).to(cdt).to(dev)
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).to(dev)
+    wd = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
+    bd = torch.from_numpy(np.ascontiguousarray(bext)).to(dev)
+    out = {k: torch.empty(n, dtype=torch.float64, device=dev)
+           for k, n in [("F", M * S), ("G", N * S), ("b", N), ("gA", M * S), ("gw", S),
+                        ("loss", 1)]}
+    it, er, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    st = torch.cuda.current_stream().cuda_stream
+    print(f"rwig_barycenter_log_ck_dev precision {precision}", flush=True)
+    check(lib().rwig_barycenter_log_ck_dev(
+        out["F"].data_ptr(), out["G"].data_ptr(), out["b"].data_ptr(), out["gA"].data_ptr(),
+        out["gw"].data_ptr(), out["loss"].data_ptr(), ctypes.byref(it), ctypes.byref(er),
+        Ad.data_ptr(), wd.data_ptr(), Cd.data_ptr(), bd.data_ptr(), M, N, S, reg, 1, L, 0.0, 7,
+        1, precision, ctypes.byref(rs), st))
+    h = {k: v.cpu().numpy() for k, v in out.items()}
+    assert it.value == r["iter"] == L
+    assert er.value == r["err"]
+    assert np.array_equal(h["b"], r["b"])
+    assert np.array_equal(h["F"].reshape(S, M).T, r["F"])
+    assert np.array_equal(h["G"].reshape(S, N).T, r["G"])
+    assert np.array_equal(h["gA"].reshape(S, M).T, r["grad_A"])
+    assert np.array_equal(h["gw"], r["grad_w"])
+    assert h["loss"][0] == r["loss"]
+    # outputs may be null
+    check(lib().rwig_barycenter_log_ck_dev(
+        None, None, out["b"].data_ptr(), None, None, None, ctypes.byref(it), ctypes.byref(er),
+        Ad.data_ptr(), wd.data_ptr(), Cd.data_ptr(), bd.data_ptr(), M, N, S, reg, 1, L, 0.0, 7,
+        1, precision, ctypes.byref(rs), st))
+    assert np.array_equal(out["b"].cpu().numpy(), r["b"])

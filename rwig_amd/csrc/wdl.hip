@@ -168,11 +168,15 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
   auto fwd = [&](int l) {
     T *kvb = KVB(l), *ub1 = UB(l + 1), *ktub1 = KTUB(l + 1), *vb1 = VB(l + 1);
     T *bb1 = bB(l + 1);
-    // KVB = K VB_l ; UB_{l+1} = A / KVB
-    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, VB(l), ldn, [=] __device__(int i, int c, T v) {
-      const long k = i + (long)c * ldn;
-      kvb[k] = v;
-      ub1[k] = A[i + (long)(c % S) * N] / v;
+    // KVB = K VB_l ; UB_{l+1} = A / KVB. The product only stores its result,
+    // and the division runs in its own streaming kernel: at the end of a
+    // product tile its loads and divisions held every workgroup's tail (the
+    // same arithmetic, so the same values)
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, VB(l), ldn,
+                          [=] __device__(int i, int c, T v) { kvb[i + (long)c * ldn] = v; });
+    launch_map(st, (long)N * SD, [=] __device__(long k) {
+      const long i = k % N, c = k / N, kk = i + c * ldn;
+      ub1[kk] = A[i + (c % S) * N] / kvb[kk];
     });
     // KTUB = K^T UB_{l+1}
     gemm_tn<T, WdlCfg<T>>(st, N, SD, N, K, ldk, ub1, ldn,
@@ -211,7 +215,7 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
   }
   // ------------------------------------------------------------ adjoint
   T *ABbar = wb.make(MSD), *Tw = wb.make(MSD), *Xb = wb.make(MSD), *tmp = wb.make(MSD);
-  T *VBbar = wb.make(MSD);
+  T *VBbar = wb.make(MSD), *UBbar = wb.make(MSD);
   launch_map(st, MSD, [=] __device__(long k) {
     ABbar[k] = T(0);
     Tw[k] = T(0);
@@ -257,10 +261,14 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
       });
     }
     // UBbar = K tmp ; ABbar += UBbar / KVB_{l-1} ; Xb = (UBbar .* UB_l) / KVB_{l-1}
-    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, tmp, ldn, [=] __device__(int i, int c, T v) {
-      const long k = i + (long)c * ldn;
-      ABbar[k] += v / kvblm1[k];
-      Xb[k] = (v * ubl[k]) / kvblm1[k];
+    // (the product stores UBbar; the update streams over it afterwards)
+    gemm_tn<T, WdlCfg<T>>(st, N, SD, N, KT, ldk, tmp, ldn,
+                          [=] __device__(int i, int c, T v) { UBbar[i + (long)c * ldn] = v; });
+    launch_map(st, (long)N * SD, [=] __device__(long k) {
+      const long i = k % N, c = k / N, kk = i + c * ldn;
+      const T v = UBbar[kk];
+      ABbar[kk] += v / kvblm1[kk];
+      Xb[kk] = (v * ubl[kk]) / kvblm1[kk];
     });
   }
   // ------------------------------------------------------------ softmax Jacobians

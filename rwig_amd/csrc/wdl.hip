@@ -23,6 +23,22 @@ namespace rwig {
 // tiles of the WDL products (S * D columns: the wide FP64 tiles)
 template <typename T> using WdlCfg = typename WideCfg<T>::type;
 
+// prod_s x[s * ld]^w[s] (the barycenter of src/wdl_impl.cpp:142-150) as
+// exp2(sum_s w_s log2 x_s): one exponential in place of S powers, the same
+// value up to rounding and without the partial products' underflow. A zero
+// weight contributes x^0 = 1, also at x = 0.
+__device__ __forceinline__ float log2_t(float x) { return log2f(x); }
+__device__ __forceinline__ double log2_t(double x) { return log2(x); }
+__device__ __forceinline__ float exp2_t(float x) { return exp2f(x); }
+__device__ __forceinline__ double exp2_t(double x) { return exp2(x); }
+template <typename T>
+__device__ __forceinline__ T weighted_geomean(const T *x, long ld, const T *w, int S) {
+  T l = T(0);
+  for (int s = 0; s < S; ++s)
+    if (w[s] != T(0)) l += w[s] * log2_t(x[s * ld]);
+  return exp2_t(l);
+}
+
 namespace {
 
 template <typename T> struct WdlBuf {
@@ -184,8 +200,7 @@ void wdl_batch_grad(hipStream_t st, const T *K, const T *KT, long ldk, int N, in
     // bB = prod_s KTUB^w ; VB = bB / KTUB
     launch_map(st, (long)N * D, [=] __device__(long k) {
       const long i = k % N, d = k / N;
-      T p = T(1);
-      for (int s = 0; s < S; ++s) p *= pow(ktub1[i + (d * S + s) * ldn], Wb[s + d * S]);
+      const T p = weighted_geomean(ktub1 + i + d * S * ldn, ldn, Wb + d * S, S);
       bb1[k] = p;
       for (int s = 0; s < S; ++s) {
         const long kk = i + (d * S + s) * ldn;
@@ -402,8 +417,7 @@ void wdl_infer(hipStream_t st, const T *K, const T *KT, long ldk, int N, int S, 
     launch_map(st, (long)N * Dn, [=] __device__(long k) {
       const long i = k % N, d = k / N;
       if (done[d]) return;
-      T p = T(1);
-      for (int s = 0; s < S; ++s) p *= pow(KTU[i + (d * S + s) * ldn], Wn[s + d * S]);
+      const T p = weighted_geomean(KTU + i + d * S * ldn, ldn, Wn + d * S, S);
       bb[k] = p;
       for (int s = 0; s < S; ++s) V[i + (d * S + s) * ldn] = p / KTU[i + (d * S + s) * ldn];
     });

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 
@@ -113,16 +114,19 @@ struct GemmShape {
 
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : GemmShape<float, 128, 128, 16, 4, 2, 2> {};
-// BF16: 16 wavefronts (4 x 4, 32 x 32 each), 3 k-steps of direct-to-LDS
-// moves in flight (4 LDS stages, 128 KB)
-template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 4, 4, 3, 1> {};
+// BF16: 8 wavefronts (2 x 4, 64 x 32 each), 3 k-steps of direct-to-LDS
+// moves in flight (4 LDS stages, 128 KB). 16 wavefronts of 32 x 32 read a
+// third more LDS bytes per flop (128 against 96 KB of fragments per k-step)
+// and ran 35.0 against 33.3 us at 4096 x 1024 x 4096 with the same loop.
+template <> struct GemmCfg<uint16_t> : GemmShape<uint16_t, 128, 128, 64, 16, 2, 4, 3, 1> {};
 template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
 
-// FP64 products with many columns (>= 256): 128 x 128 tiles, 16 wavefronts,
-// direct-to-LDS moves three k-steps ahead (60.8 TFLOP/s on 4096 x 1024 x 4096
-// against 55 for the 64 x 64 register tiles). Each output element sees the
-// same MFMA sequence as with GemmCfg<double>, so results are identical.
-using GemmF64Wide = GemmShape<double, 128, 128, 16, 2, 4, 4, 3, 1>;
+// FP64 products with many columns: 128 x 128 tiles, 8 wavefronts (2 x 4),
+// direct-to-LDS moves three k-steps ahead (72.7 TFLOP/s on 4096 x 1024 x 4096,
+// 0.92 of the FP64 MFMA peak, against 64.7 with 16 wavefronts and 55 for the
+// 64 x 64 register tiles). Each output element sees the same MFMA sequence as
+// with GemmCfg<double>, so results are identical.
+using GemmF64Wide = GemmShape<double, 128, 128, 16, 2, 2, 4, 3, 1>;
 // the tiles of a product over T with many columns
 template <typename T> struct WideCfg { using type = GemmCfg<T>; };
 template <> struct WideCfg<double> { using type = GemmF64Wide; };
@@ -432,6 +436,12 @@ __device__ __forceinline__ void wait_lgkmcnt0() {
   __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14));
 }
 
+// f(integral_constant<int, I>) for I = 0 .. sizeof...(I) - 1, in order
+template <int... I, class F>
+__device__ __forceinline__ void unroll_seq(std::integer_sequence<int, I...>, F &&f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
 // Main loop, direct-to-LDS path (CF::DMA): each global_load_lds_dwordx4
 // moves 1 KB (64 lanes x 16 bytes, written to LDS in lane order): 8 tile rows
 // of 64 k or 4 of 128 k. The lane with slot s of tile row r loads the 16-byte
@@ -445,13 +455,24 @@ __device__ __forceinline__ void wait_lgkmcnt0() {
 // for its fragment reads of step t, one barrier, then the moves of step
 // t + PF + 1 into the stage of step t (every wavefront holds step t's
 // fragments in registers by then), the fragment reads of step t + 1 into the
-// other set, and the MFMAs of step t, which the reads overlap. Each output
-// element sees the MFMAs of the register path in the same order.
+// other set, and the MFMAs of step t. Each output element sees the MFMAs of
+// the register path in the same order.
+//
+// Away from the end of the k range the loop is unrolled over a whole cycle of
+// stages, so that every stage offset is a constant (the fragment reads take
+// immediate LDS offsets and no address arithmetic), and the step's
+// instructions are issued interleaved: one MFMA, then one move, for each of
+// the wavefront's moves; then one MFMA, then one fragment read; then the rest.
+// A wavefront then keeps the matrix core fed while its moves and reads go
+// out, which matters most with one or two wavefronts per SIMD (BF16 4096 x
+// 1024 x 4096 on 128 x 128 tiles: 37.8 -> 33.3 us with 8 wavefronts instead
+// of 16; FP64: 531 -> 472 us).
 template <typename T, class CF, class Acc>
 __device__ __forceinline__ void gemm_main_dma(
     int M, int N, const T *__restrict__ A, long lda, const T *__restrict__ B, long ldb,
     const int *__restrict__ nmap, int m0, int n0, int kbeg, int kend, Acc &acc) {
   using MF = Mfma<T>;
+  using frag_t = typename MF::frag_t;
   constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK;
   constexpr int WM = CF::WM, WN = CF::WN, TM = CF::TM, TN = CF::TN;
   constexpr int NW = CF::WGM * CF::WGN;
@@ -464,10 +485,12 @@ __device__ __forceinline__ void gemm_main_dma(
   constexpr int MA = BM / NW / RPM, MB = BN / NW / RPM;  // moves per wavefront and k-step
   constexpr int MOVES = MA + MB;
   constexpr int KS = BK / MF::KI;  // MFMA k-slices per step
+  constexpr int NM = KS * TM * TN, NR = KS * (TM + TN);  // MFMAs and fragment reads per step
   static_assert(PF * MOVES < 64, "the moves in flight must fit vmcnt");
   __shared__ __attribute__((aligned(1024))) char smem[STAGES * STAGEB];
   RWIG_LDS char *lds = (RWIG_LDS char *)smem;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wv % CF::WGM, wn = wv / CF::WGM;
   // the lane's row within the move and its LDS slot; the chunk it loads is
   // slot ^ (tile row % 8)
@@ -486,8 +509,8 @@ __device__ __forceinline__ void gemm_main_dma(
   }
   const int nst = (kend - kbeg) / BK;
   if (nst <= 0) return;
-  auto issue = [&](int s) {
-    RWIG_LDS char *st = lds + (s % STAGES) * STAGEB;
+  auto issue = [&](int stage, int s) {
+    RWIG_LDS char *st = lds + stage * STAGEB;
     const int k = kbeg + s * BK;
 #pragma unroll
     for (int q = 0; q < MA; ++q)
@@ -496,32 +519,43 @@ __device__ __forceinline__ void gemm_main_dma(
     for (int q = 0; q < MB; ++q)
       lds_move16(pb[q] + k, (RWIG_LDS void *)(st + (BM + (wv * MB + q) * RPM) * ROWB));
   };
+  // byte offset within a stage of the lane's fragment of tile row `row` at
+  // k-slice ks: chunk k / EPC (at its swizzled slot), element k % EPC in it
   const int lr = lane & 15, lk = (lane >> 4) * MF::KL;  // the lane's row and first k
-  // fragment of tile row `row` at k: chunk k / EPC (at its swizzled slot),
-  // element k % EPC within it
-  auto frag = [&](const RWIG_LDS char *tile, int row, int k) -> typename MF::frag_t {
-    return *reinterpret_cast<const RWIG_LDS typename MF::frag_t *>(
-        tile + row * ROWB + (((k / EPC) ^ (row & 7)) << 4) + (k % EPC) * (int)sizeof(T));
+  auto foff = [&](int row, int ks) {
+    const int k = ks * MF::KI + lk;
+    return row * ROWB + (((k / EPC) ^ (row & 7)) << 4) + (k % EPC) * (int)sizeof(T);
   };
-  typename MF::frag_t fa[2][KS][TM], fb[2][KS][TN];
-  auto read = [&](int set, int s) {
-    const RWIG_LDS char *as = lds + (s % STAGES) * STAGEB, *bs = as + BM * ROWB;
+  int oa[KS][TM], ob[KS][TN];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) oa[ks][i] = foff(wm * WM + i * 16 + lr, ks);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) ob[ks][j] = foff(BM + wn * WN + j * 16 + lr, ks);
+  }
+  frag_t fa[2][KS][TM], fb[2][KS][TN];
+  auto read = [&](auto set_c, int stage) {
+    constexpr int SET = decltype(set_c)::value;
+    const RWIG_LDS char *st = lds + stage * STAGEB;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const int k = ks * MF::KI + lk;
 #pragma unroll
-      for (int i = 0; i < TM; ++i) fa[set][ks][i] = frag(as, wm * WM + i * 16 + lr, k);
+      for (int i = 0; i < TM; ++i)
+        fa[SET][ks][i] = *reinterpret_cast<const RWIG_LDS frag_t *>(st + oa[ks][i]);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) fb[set][ks][j] = frag(bs, wn * WN + j * 16 + lr, k);
+      for (int j = 0; j < TN; ++j)
+        fb[SET][ks][j] = *reinterpret_cast<const RWIG_LDS frag_t *>(st + ob[ks][j]);
     }
   };
-  auto mma = [&](int set) {
+  auto mma = [&](auto set_c) {
+    constexpr int SET = decltype(set_c)::value;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], fa[set][ks][i], fb[set][ks][j]);
+        for (int j = 0; j < TN; ++j) MF::mma(acc[i][j], fa[SET][ks][i], fb[SET][ks][j]);
   };
   auto barrier = [] {
     asm volatile("" ::: "memory");
@@ -530,26 +564,40 @@ __device__ __forceinline__ void gemm_main_dma(
   };
 #pragma unroll
   for (int q = 0; q < STAGES; ++q)
-    if (q < nst) issue(q);
+    if (q < nst) issue(q, q);
   wait_later<MOVES, PF>(min(nst, STAGES) - 1);
   barrier();
-  read(0, 0);
-  // Steps away from the end take compile-time waits and no branches: with no
-  // join between the reads of step t + 1 and the MFMAs of step t, the
-  // compiler does not wait for those reads before the MFMAs. The scheduling
-  // barriers keep the reads ahead of the MFMAs and the MFMAs ahead of the
-  // next step's wait.
-  auto steady = [&](auto set_c, int t) {
-    constexpr int SET = decltype(set_c)::value;
+  read(std::integral_constant<int, 0>{}, 0);
+  // Steady step t (stage S = t % STAGES, register set SET = t % 2, both
+  // compile-time): no branches, and the scheduling groups below fix the
+  // order of the step's MFMAs, moves and fragment reads.
+  auto steady = [&](auto s_c, auto set_c, int t) {
+    constexpr int S = decltype(s_c)::value, SET = decltype(set_c)::value;
     wait_vmcnt<(PF - 1) * MOVES>();
     wait_lgkmcnt0();
     barrier();
-    issue(t + STAGES);
-    read(SET ^ 1, t + 1);
     __builtin_amdgcn_sched_barrier(0);
-    mma(SET);
+    issue(S, t + STAGES);
+    read(std::integral_constant<int, SET ^ 1>{}, (S + 1) % STAGES);
+    mma(set_c);
+    constexpr int P1 = MOVES < NM ? MOVES : NM;                // MFMA + move
+    constexpr int P2 = NR < NM - P1 ? NR : NM - P1;            // MFMA + read
+#pragma unroll
+    for (int p = 0; p < P1; ++p) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load (into LDS)
+    }
+#pragma unroll
+    for (int p = 0; p < P2; ++p) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+    }
+    if constexpr (NM - P1 - P2 > 0) __builtin_amdgcn_sched_group_barrier(0x008, NM - P1 - P2, 0);
+    if constexpr (NR - P2 > 0) __builtin_amdgcn_sched_group_barrier(0x100, NR - P2, 0);
+    if constexpr (MOVES - P1 > 0) __builtin_amdgcn_sched_group_barrier(0x020, MOVES - P1, 0);
     __builtin_amdgcn_sched_barrier(0);
   };
+  // the last steps: stages known at run time, waits by the moves left
   auto last = [&](auto set_c, int t) {
     constexpr int SET = decltype(set_c)::value;
     if (t >= nst) return;
@@ -557,21 +605,23 @@ __device__ __forceinline__ void gemm_main_dma(
       wait_later<MOVES, PF - 1>(min(nst, t + STAGES) - (t + 2));
       wait_lgkmcnt0();
       barrier();
-      if (t + STAGES < nst) issue(t + STAGES);
-      read(SET ^ 1, t + 1);
+      if (t + STAGES < nst) issue(t % STAGES, t + STAGES);
+      read(std::integral_constant<int, SET ^ 1>{}, (t + 1) % STAGES);
     }
-    mma(SET);
+    mma(set_c);
   };
-  using Set0 = std::integral_constant<int, 0>;
-  using Set1 = std::integral_constant<int, 1>;
+  // a whole number of stage cycles and register-set pairs per unrolled block
+  constexpr int U = STAGES % 2 == 0 ? STAGES : 2 * STAGES;
   int t = 0;
-  for (; t + 1 + STAGES < nst; t += 2) {
-    steady(Set0{}, t);
-    steady(Set1{}, t + 1);
-  }
+  for (; t + U - 1 + STAGES < nst; t += U)
+    unroll_seq(std::make_integer_sequence<int, U>{}, [&](auto i_c) {
+      constexpr int I = decltype(i_c)::value;
+      steady(std::integral_constant<int, I % STAGES>{}, std::integral_constant<int, I % 2>{},
+             t + I);
+    });
   for (; t < nst; t += 2) {
-    last(Set0{}, t);
-    last(Set1{}, t + 1);
+    last(std::integral_constant<int, 0>{}, t);
+    last(std::integral_constant<int, 1>{}, t + 1);
   }
 }
 

@@ -91,7 +91,7 @@ template <> struct Mfma<float> {
 // (global_load_lds_dwordx4, no register staging) through PF + 1 LDS stages, PF
 // k-steps ahead, in an unpadded layout whose 16-byte chunks are XOR-swizzled
 // by row (PAD is then unused). Needs tile rows of 128 or 256 bytes (BF16: 64
-// or 128 k; FP64: 16 or 32 k), K a multiple of BK and 16-byte aligned rows;
+// or 128 k; FP32: 32 or 64 k; FP64: 16 or 32 k) and 16-byte aligned rows;
 // gemm_tn falls back to the register path otherwise.
 template <typename T, int BM_, int BN_, int BK_, int PAD_, int WGM_, int WGN_, int PF_ = 1,
           int DMA_ = 0>
@@ -127,9 +127,15 @@ template <> struct GemmCfg<double> : GemmShape<double, 64, 64, 16, 2, 2, 2> {};
 // 64 x 64 register tiles). Each output element sees the same MFMA sequence as
 // with GemmCfg<double>, so results are identical.
 using GemmF64Wide = GemmShape<double, 128, 128, 16, 2, 2, 4, 3, 1>;
+// FP32 products with many columns (the WDL step's 2000 x 8192 x 2000):
+// 128 x 128 tiles, 8 wavefronts, direct-to-LDS moves two k-steps ahead, 32 k
+// per step (482.7 us against 534.9 us for the 4-wavefront register tiles of
+// GemmCfg<float>, same MFMA sequence per output element).
+using GemmF32Wide = GemmShape<float, 128, 128, 32, 4, 2, 4, 2, 1>;
 // the tiles of a product over T with many columns
 template <typename T> struct WideCfg { using type = GemmCfg<T>; };
 template <> struct WideCfg<double> { using type = GemmF64Wide; };
+template <> struct WideCfg<float> { using type = GemmF32Wide; };
 
 // Loads the BM x BK (or BN x BK) tile rows starting at (r0, k0) of a matrix
 // stored with its k values contiguous (element (r, k) at p[r * ld + k]) into
@@ -256,7 +262,8 @@ inline size_t splitk_work_elems(int M, int N, int ks) {
 //        <= 512: 64 x 64, 4 waves     31-34     <= 768: 64 x 64, 4 waves   495-510
 //        above: 128 x 128, 16 waves   44-50     above: 128 x 128, 16 waves 560-590
 // (the 128 x 128 tiles alone take 37-42 us in BF16 and 530-555 us in FP64 at
-// any count up to 512 columns). FP32 products keep GemmCfg<float>.
+// any count up to 512 columns). FP32 products keep GemmCfg<float> here; the
+// WDL step takes GemmF32Wide.
 using Bf16Micro = GemmShape<uint16_t, 16, 32, 64, 16, 1, 2, 3, 1>;
 using Bf16Tiny = GemmShape<uint16_t, 32, 32, 64, 16, 2, 2, 3, 1>;
 using Bf16Small = GemmShape<uint16_t, 32, 32, 64, 16, 1, 1, 2, 1>;
@@ -507,8 +514,7 @@ __device__ __forceinline__ void gemm_main_dma(
     const long r = n < N ? (nmap ? nmap[n] : n) : (nmap ? nmap[0] : 0);
     pb[q] = B + r * ldb + (slot ^ (rl & 7)) * EPC;
   }
-  const int nst = (kend - kbeg) / BK;
-  if (nst <= 0) return;
+  const int nst = (kend - kbeg) / BK;  // whole k-steps; a shorter last one follows
   auto issue = [&](int stage, int s) {
     RWIG_LDS char *st = lds + stage * STAGEB;
     const int k = kbeg + s * BK;
@@ -562,6 +568,49 @@ __device__ __forceinline__ void gemm_main_dma(
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
   };
+  // The last, partial k-step (k range not a multiple of BK): the same lanes
+  // load the same 16-byte chunks through registers, k >= kend as zero, and
+  // store them where the moves would have put them; the MFMAs then run as in
+  // any other step, so each output element still sees the register path's
+  // MFMA sequence (the zero k-slices of the step add exact zeros).
+  auto tail = [&](int k0) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    auto chunk = [&](const T *row, int kc) {
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if (kc + EPC <= kend) {
+        v = *reinterpret_cast<const u32x4 *>(row);
+      } else if (kc < kend) {
+        T tmp[EPC];
+#pragma unroll
+        for (int e = 0; e < EPC; ++e) tmp[e] = kc + e < kend ? row[e] : T(0);
+        v = *reinterpret_cast<const u32x4 *>(tmp);
+      }
+      return v;
+    };
+    wait_vmcnt<0>();
+    wait_lgkmcnt0();
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MA; ++q) {
+      const int rl = (wv * MA + q) * RPM + sub;
+      const u32x4 v = chunk(pa[q] + k0, k0 + (slot ^ (rl & 7)) * EPC);
+      *reinterpret_cast<RWIG_LDS u32x4 *>(lds + (wv * MA + q) * RPM * ROWB + lane * 16) = v;
+    }
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+      const int rl = (wv * MB + q) * RPM + sub;
+      const u32x4 v = chunk(pb[q] + k0, k0 + (slot ^ (rl & 7)) * EPC);
+      *reinterpret_cast<RWIG_LDS u32x4 *>(lds + (BM + (wv * MB + q) * RPM) * ROWB + lane * 16) =
+          v;
+    }
+    __syncthreads();
+    read(std::integral_constant<int, 0>{}, 0);
+    mma(std::integral_constant<int, 0>{});
+  };
+  if (nst <= 0) {
+    if (kend > kbeg) tail(kbeg);
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < STAGES; ++q)
     if (q < nst) issue(q, q);
@@ -623,6 +672,7 @@ __device__ __forceinline__ void gemm_main_dma(
     last(std::integral_constant<int, 0>{}, t);
     last(std::integral_constant<int, 1>{}, t + 1);
   }
+  if (kbeg + nst * BK < kend) tail(kbeg + nst * BK);
 }
 
 template <typename T, class CF, class Op>
@@ -738,11 +788,12 @@ void gemm_tn(hipStream_t st, int M, int N, int K, const T *A, long lda, const T 
              Op op, const int *nmap = nullptr, const int *ncount = nullptr,
              const SplitK<T> &sk = SplitK<T>(), bool epilogue = true) {
   if constexpr (CF::DMA) {
-    // the direct-to-LDS path moves whole 16-byte chunks of whole k-steps: other
-    // shapes take the register path with the same tiles (same arithmetic,
-    // same split workspace)
+    // the direct-to-LDS path moves 16-byte chunks of 16-byte aligned rows (a
+    // ragged last k-step goes through registers): other operands take the
+    // register path with the same tiles (same arithmetic, same split
+    // workspace)
     constexpr long EPC = 16 / sizeof(T);
-    const bool ok = K % CF::BK == 0 && lda % EPC == 0 && ldb % EPC == 0 &&
+    const bool ok = lda % EPC == 0 && ldb % EPC == 0 &&
                     (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
                     (reinterpret_cast<uintptr_t>(B) & 15) == 0;
     if (!ok) {

@@ -34,8 +34,8 @@ def run_bf16(A, Bt, M, N, K, ks=1, deferred=False):
 
 
 # (ks, deferred): fused epilogue; deferred epilogue; k split in 2 and 5.
-# Shapes with K a multiple of 64 take the direct-to-LDS path, the others the
-# register path.
+# Rows 16-byte aligned take the direct-to-LDS path; when K is not a multiple
+# of the k-step, its last, partial step goes through registers.
 SPLITS = [(1, False), (1, True), (2, True), (5, True)]
 
 
@@ -104,10 +104,12 @@ def test_bf16_gemm_throughput_report(ks, deferred):
 
 
 @pytest.mark.parametrize("ks,deferred", [(1, False), (2, True)])
-@pytest.mark.parametrize("M,N,K", [(300, 70, 256), (1000, 257, 1024)])
+@pytest.mark.parametrize("M,N,K", [(300, 70, 256), (1000, 257, 1024), (300, 70, 200),
+                                   (1000, 257, 1000), (129, 65, 40)])
 def test_bf16_direct_to_lds_equals_register_path(M, N, K, ks, deferred):
-    # the same operands once 16-byte aligned (direct-to-LDS path) and once
-    # shifted by 4 elements (rows no longer 16-byte aligned: register path)
+    # the same operands once 16-byte aligned (direct-to-LDS path, with a
+    # partial last k-step when K % 64 != 0, only that step when K < 64) and
+    # once shifted by 4 elements (rows no longer 16-byte aligned: register path)
     g = torch.Generator(device="cpu").manual_seed(7 * M + K)
     lda = K + 8
     A = torch.randn((M, lda), generator=g).to(torch.bfloat16).cuda()
@@ -123,6 +125,30 @@ def test_bf16_direct_to_lds_equals_register_path(M, N, K, ks, deferred):
                                       Bs[:, 4:].data_ptr(), lda + 8, D.data_ptr(), M, ks,
                                       None if w is None else w.data_ptr(),
                                       torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert torch.equal(Da, D)
+
+
+@pytest.mark.parametrize("ks,deferred", [(1, False), (3, True)])
+@pytest.mark.parametrize("M,N,K", [(300, 900, 256), (300, 900, 201), (257, 70, 10)])
+def test_f64_direct_to_lds_equals_register_path(M, N, K, ks, deferred):
+    # as above for FP64 (k-steps of 16): rows shifted by one element are no
+    # longer 16-byte aligned and take the register path
+    g = torch.Generator(device="cpu").manual_seed(5 * M + K)
+    lda = K + 2 + K % 2
+    A = torch.randn((M, lda), generator=g, dtype=torch.float64).cuda()
+    Bt = torch.randn((N, lda), generator=g, dtype=torch.float64).cuda()
+    Da = run_f64(A, Bt, M, N, K, ks, deferred)
+    As = torch.empty((M, lda + 2), dtype=torch.float64, device="cuda")
+    Bs = torch.empty((N, lda + 2), dtype=torch.float64, device="cuda")
+    As[:, 1:1 + lda] = A
+    Bs[:, 1:1 + lda] = Bt
+    D = torch.empty((N, M), dtype=torch.float64, device="cuda")
+    w = workspace(1, M, N, ks) if deferred else None
+    check(lib().rwig_gemm_tn_f64_dev(M, N, K, As[:, 1:].data_ptr(), lda + 2,
+                                     Bs[:, 1:].data_ptr(), lda + 2, D.data_ptr(), M, ks,
+                                     None if w is None else w.data_ptr(),
+                                     torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     assert torch.equal(Da, D)
 
@@ -143,7 +169,8 @@ def test_every_tile_shape_gives_the_same_columns(dtype, K, ks, deferred):
     """The tile shape follows the column count (smaller tiles for fewer
     columns). A product over the first n columns must equal those columns of
     the product over all of them bit for bit, for n in every shape's range.
-    K = 256 takes the direct-to-LDS tiles, K = 200 / 201 the register path."""
+    K = 256 takes the direct-to-LDS tiles in whole k-steps, K = 200 / 201 ends
+    with a partial k-step."""
     M = 300
     g = torch.Generator(device="cpu").manual_seed(K + ks)
     if dtype == "bf16":

@@ -37,11 +37,212 @@
 
 #include <type_traits>
 
+#include "gemm.hpp"
 #include "kernels.hpp"
 
 namespace rwig {
 
 namespace {
+
+// ---------------------------------------------------------------- FP64 soft-min as a product
+//
+// With FP64 iterates and a cost whose range (max C - min C) / reg is at most
+// kScaledRange, the row soft-min of R_s(i,j) = C(i,j) - F(i,s) - G(j,s) is
+//   -F(i,s) - reg log(sum_j Kt(i,j) E(j,s)) - m_s + cmin,
+// Kt = exp(-(C - cmin) / reg), E = exp((G - m_s) / reg), m_s = max_j G(j,s)
+// (the column soft-min likewise, with F and G exchanged): one FP64 MFMA
+// product of the M x N scaled kernel with the S slices per pass, in place of
+// an exponential per entry and slice. Every factor is at most 1, and in every
+// row the term of j = argmax_j G(j,s) is at least exp(-kScaledRange), so no
+// sum underflows and the terms lost below FP64's range (exp(-744)) are under
+// exp(-144) of it; the value is the log-sum-exp form's up to rounding.
+constexpr double kScaledRange = 600.0;
+
+// part[2 b], part[2 b + 1] = min and max of block b's share of x[0 .. n);
+// with part == out and one block, the min and max of part's 2 nb entries
+__global__ void __launch_bounds__(kTB) k_minmax(const double *x, long n, double *out) {
+  __shared__ double smin[kTB / kWave], smax[kTB / kWave];
+  double lo = INFINITY, hi = -INFINITY;
+  for (long k = blockIdx.x * (long)kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
+    lo = fmin(lo, x[k]);
+    hi = fmax(hi, x[k]);
+  }
+  for (int o = kWave / 2; o > 0; o /= 2) {
+    lo = fmin(lo, __shfl_xor(lo, o));
+    hi = fmax(hi, __shfl_xor(hi, o));
+  }
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (lane == 0) {
+    smin[w] = lo;
+    smax[w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kTB / kWave; ++i) {
+      lo = fmin(lo, smin[i]);
+      hi = fmax(hi, smax[i]);
+    }
+    out[2 * blockIdx.x] = lo;
+    out[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+// min and max of C (M x N entries) into mm[0], mm[1]; part: 2 * kMinmaxBlocks
+constexpr int kMinmaxBlocks = 1024;
+void minmax(hipStream_t st, const double *C, long n, double *part, double *mm) {
+  hipLaunchKernelGGL(k_minmax, dim3(kMinmaxBlocks), dim3(kTB), 0, st, C, n, part);
+  check_launch();
+  // the 1024 partial minima and maxima, read as 2048 values: the min of all
+  // is the min of the minima (a maximum is never below its block's minimum)
+  // and likewise for the max
+  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(kTB), 0, st, part, 2L * kMinmaxBlocks, mm);
+  check_launch();
+}
+
+// m[s] = max_i X(i, s) over n rows (0 where no entry is finite), one block
+// per slice
+__global__ void __launch_bounds__(kTB)
+k_slice_max(const double *X, int n, double *m, const LoopState *ls) {
+  if (ls && !ls->active) return;
+  __shared__ double sh[kTB / kWave];
+  const int s = blockIdx.x;
+  double hi = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += kTB) hi = fmax(hi, X[i + (long)s * n]);
+  for (int o = kWave / 2; o > 0; o /= 2) hi = fmax(hi, __shfl_xor(hi, o));
+  if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = hi;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kTB / kWave; ++i) hi = fmax(hi, sh[i]);
+    m[s] = isfinite(hi) ? hi : 0.0;
+  }
+}
+
+// The scaled kernel in both layouts and the passes' scratch.
+struct ScaledSoftmin {
+  const double *Kr = nullptr;  // Kt(i, j) at Kr[j + i * N] (C^T's layout): rows
+  const double *Kc = nullptr;  // Kt(i, j) at Kc[i + j * M] (C's layout): columns
+  double cmin = 0.0;
+  double *m = nullptr, *E = nullptr, *mv = nullptr, *wr = nullptr, *wc = nullptr;
+  int ksr = 1, ksc = 1;
+};
+
+// k splits of a scaled pass: twice the split the product would take alone
+// (the passes have S <= 32 columns and stream the kernel once).
+inline int scaled_splits(int rows, int k) {
+  return std::min(8, 2 * splitk_for<double, F64Small>(rows, k));
+}
+
+// row soft-min (as lse_row): op(i + s M, value)
+template <class Op>
+void scaled_row(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
+                const double *F, const double *G, Op op, const LoopState *ls) {
+  double *m = sp.m, *E = sp.E;
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, G, N, m, ls);
+  check_launch();
+  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp((G[k] - m[k / N]) / reg); }, ls);
+  SplitK<double> sk;
+  sk.ks = sp.ksr;
+  sk.work = sk.ks > 1 ? sp.wr : nullptr;
+  const double cm = sp.cmin;
+  gemm_tn<double, F64Small>(st, M, S, N, sp.Kr, N, E, N,
+                            [=] __device__(int i, int s, double y) {
+                              if (ls && !ls->active) return;
+                              const long k = i + (long)s * M;
+                              op(k, -F[k] - reg * log(y) - m[s] + cm);
+                            },
+                            nullptr, nullptr, sk);
+}
+
+// column soft-min (as lse_col_t): op(j, values of the S slices)
+template <class Op>
+void scaled_col(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
+                const double *F, const double *G, Op op, const LoopState *ls) {
+  double *m = sp.m, *E = sp.E, *mv = sp.mv;
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, F, M, m, ls);
+  check_launch();
+  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp((F[k] - m[k / M]) / reg); }, ls);
+  SplitK<double> sk;
+  sk.ks = sp.ksc;
+  sk.work = sk.ks > 1 ? sp.wc : nullptr;
+  const double cm = sp.cmin;
+  gemm_tn<double, F64Small>(st, N, S, M, sp.Kc, M, E, M,
+                            [=] __device__(int j, int s, double z) {
+                              if (ls && !ls->active) return;
+                              const long k = j + (long)s * N;
+                              mv[k] = -G[k] - reg * log(z) - m[s] + cm;
+                            },
+                            nullptr, nullptr, sk);
+  RWIG_DISPATCH_S(S, SM,
+                  hipLaunchKernelGGL((k_cols_apply<double, SM, Op>),
+                                     dim3((N + kWave - 1) / kWave), dim3(kWave), 0, st, mv, N, S,
+                                     op, ls));
+  check_launch();
+}
+
+// z * exp(x) for a sum z of the scaled form and its exponent x in natural
+// units, as ldexp(z 2^f, n) with x log2(e) = n + f: no overflow of exp(x)
+// on the way to a representable result
+__device__ __forceinline__ double scale_exp(double z, double x) {
+  const double t = x * kLog2e, n = floor(t);
+  return ldexp(z * exp2(t - n), (int)fmax(fmin(n, 4000.0), -4000.0));
+}
+
+// The adjoint's weighted sums as scaled products (the same bound: the largest
+// weight of every row or column is at least exp(-kScaledRange) before its
+// factor). Column sums with row soft-max weights (as wcol_t):
+//   res(j,s) = sum_i exp(-(C(i,j) - F(i,s) - G(j,s) - rsh(i,s)) / reg) x(i,s)
+//            = exp(G(j,s)/reg + m_s - cmin/reg) sum_i Kt(i,j) E(i,s),
+//   E = exp(a - m_s) x, a = (F + rsh) / reg, m_s = max_i a(i,s);
+// op(j, values of the S slices).
+template <class Op>
+void scaled_wcol(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
+                 const double *F, const double *G, const double *rsh, const double *x, Op op) {
+  double *m = sp.m, *E = sp.E, *mv = sp.mv;
+  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = (F[k] + rsh[k]) / reg; });
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, E, M, m, nullptr);
+  check_launch();
+  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / M]) * x[k]; });
+  SplitK<double> sk;
+  sk.ks = sp.ksc;
+  sk.work = sk.ks > 1 ? sp.wc : nullptr;
+  const double cm = sp.cmin;
+  gemm_tn<double, F64Small>(st, N, S, M, sp.Kc, M, E, M,
+                            [=] __device__(int j, int s, double z) {
+                              const long k = j + (long)s * N;
+                              mv[k] = scale_exp(z, G[k] / reg + m[s] - cm / reg);
+                            },
+                            nullptr, nullptr, sk);
+  RWIG_DISPATCH_S(S, SM,
+                  hipLaunchKernelGGL((k_cols_apply<double, SM, Op>),
+                                     dim3((N + kWave - 1) / kWave), dim3(kWave), 0, st, mv, N, S,
+                                     op, (const LoopState *)nullptr));
+  check_launch();
+}
+
+// Row sums with column soft-max weights (as wrow with a row shift):
+//   part(i,s) = sum_j exp(-(C(i,j) - F(i,s) - G(j,s) - csh(j,s) + rowsh(i,s)) / reg) y(j,s)
+//             = exp((F - rowsh)(i,s)/reg + m_s - cmin/reg) sum_j Kt(i,j) E(j,s),
+//   E = exp(v - m_s) y, v = (G + csh) / reg, m_s = max_j v(j,s); op(i + s M, part).
+template <class Op>
+void scaled_wrow(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
+                 const double *F, const double *G, const double *csh, const double *y,
+                 const double *rowsh, Op op) {
+  double *m = sp.m, *E = sp.E;
+  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = (G[k] + csh[k]) / reg; });
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, E, N, m, nullptr);
+  check_launch();
+  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / N]) * y[k]; });
+  SplitK<double> sk;
+  sk.ks = sp.ksr;
+  sk.work = sk.ks > 1 ? sp.wr : nullptr;
+  const double cm = sp.cmin;
+  gemm_tn<double, F64Small>(st, M, S, N, sp.Kr, N, E, N,
+                            [=] __device__(int i, int s, double z) {
+                              const long k = i + (long)s * M;
+                              op(k, scale_exp(z, (F[k] - rowsh[k]) / reg + m[s] - cm / reg));
+                            },
+                            nullptr, nullptr, sk);
+}
 
 __device__ __forceinline__ void finish_iteration_ck(LoopState *ls, double err) {
   ls->err = err;
@@ -103,6 +304,56 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
   LoopState *ls = ar.make<LoopState>(1);
   double *gpart = ar.make<double>(gram_part_size(S));
   const T treg = (T)reg;
+  // FP64: the soft-min passes as products with the scaled kernel when the
+  // cost's range allows it (the same values up to rounding)
+  constexpr bool kScalable = std::is_same<T, double>::value && std::is_same<X, double>::value;
+  bool scaled = false;
+  ScaledSoftmin sp;
+  if constexpr (kScalable) {
+    double *mm = ar.make<double>(2), *mpart = ar.make<double>(2 * kMinmaxBlocks), hmm[2];
+    minmax(st, C, (long)M * N, mpart, mm);
+    d2h(hmm, mm, 2, st);
+    RWIG_HIP_CHECK(hipStreamSynchronize(st));
+    if (std::isfinite(hmm[0]) && std::isfinite(hmm[1]) && (hmm[1] - hmm[0]) / reg <= kScaledRange) {
+      scaled = true;
+      const double cmin = hmm[0];
+      double *Kc = ar.make<double>((size_t)M * N), *Kr = ar.make<double>((size_t)M * N);
+      const double *Cd = C, *CTd = CT;
+      launch_map(st, (long)M * N, [=] __device__(long k) {
+        Kc[k] = exp(-(Cd[k] - cmin) / reg);
+        Kr[k] = exp(-(CTd[k] - cmin) / reg);
+      });
+      sp.Kr = Kr;
+      sp.Kc = Kc;
+      sp.cmin = cmin;
+      sp.m = ar.make<double>(S);
+      sp.E = ar.make<double>((size_t)std::max(M, N) * S);
+      sp.mv = ar.make<double>((size_t)N * S);
+      sp.ksr = scaled_splits(M, N);
+      sp.ksc = scaled_splits(N, M);
+      sp.wr = ar.make<double>(splitk_work_elems<double, F64Small>(M, S, sp.ksr));
+      sp.wc = ar.make<double>(splitk_work_elems<double, F64Small>(N, S, sp.ksc));
+    }
+  }
+  // the forward's two soft-min passes over the working F, G
+  auto row_pass = [&](auto op, const LoopState *gate) {
+    if constexpr (kScalable) {
+      if (scaled) {
+        scaled_row(st, sp, M, N, S, reg, F, G, op, gate);
+        return;
+      }
+    }
+    lse_row<T, X>(st, C, M, N, M, F, M, G, N, S, reg, pm, ps, op, gate);
+  };
+  auto col_pass = [&](auto op, const LoopState *gate) {
+    if constexpr (kScalable) {
+      if (scaled) {
+        scaled_col(st, sp, M, N, S, reg, F, G, op, gate);
+        return;
+      }
+    }
+    lse_col_t<T, X>(st, CT, M, N, N, F, M, G, N, S, reg, cpm, cps, op, gate);
+  };
   const int ncp = wg ? maxiter / ck + 1 : 0;
   T *Fc = wg ? ar.make<T>((size_t)ncp * MS) : nullptr;
   T *Gc = wg ? ar.make<T>((size_t)ncp * NS) : nullptr;
@@ -126,8 +377,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
     h.active = (0 < maxiter && h.err >= tol) ? 1 : 0;
     h2d(ls, &h, 1, st);
   }
-  lse_row<T, X>(
-      st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
+  row_pass(
       [=] __device__(long k, T r) {
         KV[k] = r;
         if (KVc) KVc[k] = r;
@@ -156,8 +406,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         gate);
     // KTU = column soft-min of R(F, G); logb = -(G + KTU) w / reg;
     // G += reg logb 1^T + KTU
-    lse_col_t<T, X>(
-        st, CT, M, N, N, F, M, G, N, S, reg, cpm, cps,
+    col_pass(
         [=] __device__(int j, const T *r) {
           T acc = T(0);
           for (int s = 0; s < S; ++s) acc += -(G[j + (long)s * N] + r[s]) * w[s];
@@ -178,8 +427,7 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         },
         gate);
     // KV = row soft-min of R(F, G)
-    lse_row<T, X>(
-        st, C, M, N, M, F, M, G, N, S, reg, pm, ps,
+    row_pass(
         [=] __device__(long k, T r) {
           KV[k] = r;
           if (gate) {
@@ -302,17 +550,23 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         const T *KVl = hKV + q * MS, *KTUl = hKTU + q * NS;
         if (l != L) {
           // Gbar(:,s) = -X_s^T Fbar(:,s); logbbar = reg * rowsum(Gbar)
-          wcol_t<T, X>(
-              st, CT, M, N, N, Fl, M, Gl, N, S, reg, KVl, Fbar, cpart,
-              [=] __device__(int j, const T *r) {
-                T acc = T(0);
-                for (int s = 0; s < S; ++s) {
-                  Gbar[j + (long)s * N] = -r[s];
-                  acc += -r[s];
-                }
-                lbb[j] = treg * acc;
-              },
-              nullptr);
+          auto gop = [=] __device__(int j, const T *r) {
+            T acc = T(0);
+            for (int s = 0; s < S; ++s) {
+              Gbar[j + (long)s * N] = -r[s];
+              acc += -r[s];
+            }
+            lbb[j] = treg * acc;
+          };
+          bool done = false;
+          if constexpr (kScalable) {
+            if (scaled) {
+              scaled_wcol(st, sp, M, N, S, reg, Fl, Gl, KVl, Fbar, gop);
+              done = true;
+            }
+          }
+          if (!done) wcol_t<T, X>(st, CT, M, N, N, Fl, M, Gl, N, S, reg, KVl, Fbar, cpart, gop,
+                                  nullptr);
         } else {
           launch_map(st, (long)N, [=] __device__(long j) { lbb[j] = (T)seedv[j]; });
         }
@@ -327,13 +581,19 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
         });
         // computed relative to each row's A (Q = Fbar / A, rows shifted by
         // reg log A): rows of A far below the FP32 range keep their gradient
-        wrow<T, X>(
-            st, C, M, N, M, Fl, M, Glm1, N, S, reg, KTUl, Y, part,
-            [=] __device__(long k, T q) {
-              Fbar[k] = (T)((double)q * A[k]);
-              gA[k] += (double)q;
-            },
-            nullptr, rlogA);
+        auto fop = [=] __device__(long k, T q) {
+          Fbar[k] = (T)((double)q * A[k]);
+          gA[k] += (double)q;
+        };
+        bool done = false;
+        if constexpr (kScalable) {
+          if (scaled) {
+            scaled_wrow(st, sp, M, N, S, reg, Fl, Glm1, KTUl, Y, rlogA, fop);
+            done = true;
+          }
+        }
+        if (!done) wrow<T, X>(st, C, M, N, M, Fl, M, Glm1, N, S, reg, KTUl, Y, part, fop, nullptr,
+                              rlogA);
       }
       l1 = l0;
     }

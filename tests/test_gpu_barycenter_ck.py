@@ -15,6 +15,11 @@ Bars:
   ~1e-8).
 - the log form's "unnormalised" gradient equals the golden-pinned parallel
   barycenter gradient of the oracle (1e-9).
+- FP64 soft-min passes and the adjoint's weighted sums: as products with the
+  scaled kernel exp(-(C - min C) / reg) when (max C - min C) / reg <= 600, as
+  log-sum-exp passes otherwise; the two forms agree at the C3 size (1e-12 on
+  b, F, G and the loss, 1e-10 on grad_w and on grad_A where A >= 1e-9), and
+  the wide-range fallback matches the oracle to 1e-10.
 """
 import numpy as np
 import pytest
@@ -51,6 +56,45 @@ def test_fp64_reference_seed_matches_plain_and_oracle(seed, M, N, S, reg, every)
     assert r["return_status"] == p["return_status"]
     for k in ["b", "F", "G", "grad_A", "grad_w", "loss"]:
         assert rel(r[k], p[k]) < 1e-12, k
+        assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+def test_c3_fp64_scaled_products_equal_the_log_sum_exp_passes():
+    # C3's cost spans 1 / 0.004 = 250 units of reg: the checkpointed form runs
+    # its soft-min passes as scaled FP64 products, the plain log barycenter as
+    # log-sum-exp passes; 200 fixed iterations
+    A, C, w, bext, eps, L = c3_inputs()
+    r = rwig_amd.barycenter_log_ck(A, C, w, bext, reg=eps, max_iter=L, zero_tol=0.0,
+                                   with_grad=True)
+    p = rwig_amd.barycenter(A, C, w, bext, {"reg": eps, "method": "log", "max_iter": L,
+                                             "zero_tol": 0.0, "with_grad": True})
+    assert r["iter"] == p["iter"] == L
+    errs = {k: rel(r[k], p[k]) for k in ["b", "F", "G", "loss", "grad_w"]}
+    # grad_A where it is resolvable in FP64 (A >= 1e-9: one ulp of C moves it
+    # by ~1e-14 there, by ~1e-8 where A < 1e-20, as the test below shows)
+    big = A >= 1e-9
+    d = r["grad_A"] - p["grad_A"]
+    errs["grad_A(A>=1e-9)"] = np.max(np.abs(d[big])) / np.max(np.abs(p["grad_A"][big]))
+    bands = _band_errors(A, r["grad_A"], p["grad_A"])
+    print("C3 FP64 scaled products against log-sum-exp passes: "
+          + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()) + "; grad_A by band of A: "
+          + ", ".join(f"[{lo:g},{hi:g}) {e:.1e}" for (lo, hi), e in bands.items()))
+    for k in ["b", "F", "G", "loss"]:
+        assert errs[k] < 1e-12, k
+    assert errs["grad_w"] < 1e-10 and errs["grad_A(A>=1e-9)"] < 1e-10
+
+
+def test_fp64_wide_cost_range_keeps_the_log_sum_exp_passes():
+    # (max C - min C) / reg = 1,000 or more: the scaled kernel would leave the
+    # FP64 range, so the log-sum-exp passes run; results against the oracle
+    A, C, w, bext = bary_problem(57, 40, 50, 3)
+    C = C * (40.0 / C.max())
+    reg = 0.04
+    r = rwig_amd.barycenter_log_ck(A, C, w, bext, reg=reg, with_grad=True, max_iter=300,
+                                   zero_tol=1e-8, checkpoint_every=9)
+    o = O.barycenter_log(A, C, w, bext, reg, withgrad=True, maxiter=300, zerotol=1e-8)
+    assert r["iter"] == o["iter"]
+    for k in ["b", "F", "G", "grad_A", "grad_w", "loss"]:
         assert rel(r[k], o[k]) < TOL_FP64, k
 
 

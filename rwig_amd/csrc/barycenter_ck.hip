@@ -99,6 +99,16 @@ void minmax(hipStream_t st, const double *C, long n, double *part, double *mm) {
   check_launch();
 }
 
+// *flag = 1 if some entry of x differs from the one of y (x, y: n entries;
+// *flag starts at 0)
+__global__ void __launch_bounds__(kTB) k_differ(const double *x, const double *y, long n,
+                                                  int *flag) {
+  bool d = false;
+  for (long k = blockIdx.x * (long)kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB)
+    d = d || x[k] != y[k];
+  if (__any(d) && (threadIdx.x % kWave) == 0) atomicOr(flag, 1);
+}
+
 // m[s] = max_i X(i, s) over n rows (0 where no entry is finite), one block
 // per slice
 __global__ void __launch_bounds__(kTB)
@@ -313,15 +323,28 @@ BaryCkResult barycenter_log_ck(hipStream_t st, const double *A, const T *C, cons
     double *mm = ar.make<double>(2), *mpart = ar.make<double>(2 * kMinmaxBlocks), hmm[2];
     minmax(st, C, (long)M * N, mpart, mm);
     d2h(hmm, mm, 2, st);
+    // a symmetric cost (C = C^T, as on a grid) keeps one scaled kernel for
+    // both passes: half the memory, and the passes then stream the same
+    // 8 M N bytes
+    int *dflag = ar.make<int>(1), hdiff = 1;
+    if (M == N) {
+      launch_map(st, 1L, [=] __device__(long) { *dflag = 0; });
+      hipLaunchKernelGGL(k_differ, dim3(1024), dim3(kTB), 0, st, (const double *)C,
+                         (const double *)CT, (long)M * N, dflag);
+      check_launch();
+      d2h(&hdiff, dflag, 1, st);
+    }
     RWIG_HIP_CHECK(hipStreamSynchronize(st));
     if (std::isfinite(hmm[0]) && std::isfinite(hmm[1]) && (hmm[1] - hmm[0]) / reg <= kScaledRange) {
       scaled = true;
       const double cmin = hmm[0];
-      double *Kc = ar.make<double>((size_t)M * N), *Kr = ar.make<double>((size_t)M * N);
+      double *Kc = ar.make<double>((size_t)M * N);
+      double *Kr = hdiff ? ar.make<double>((size_t)M * N) : Kc;
       const double *Cd = C, *CTd = CT;
       launch_map(st, (long)M * N, [=] __device__(long k) {
-        Kc[k] = exp(-(Cd[k] - cmin) / reg);
-        Kr[k] = exp(-(CTd[k] - cmin) / reg);
+        const double kc = exp(-(Cd[k] - cmin) / reg);
+        Kc[k] = kc;
+        if (Kr != Kc) Kr[k] = exp(-(CTd[k] - cmin) / reg);
       });
       sp.Kr = Kr;
       sp.Kc = Kc;

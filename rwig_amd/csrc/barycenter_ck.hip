@@ -110,19 +110,20 @@ __global__ void __launch_bounds__(kTB) k_differ(const double *x, const double *y
 }
 
 // m[s] = max_i X(i, s) over n rows (0 where no entry is finite), one block
-// per slice
-__global__ void __launch_bounds__(kTB)
+// of 1024 threads per slice
+constexpr int kMaxTB = 1024;
+__global__ void __launch_bounds__(kMaxTB)
 k_slice_max(const double *X, int n, double *m, const LoopState *ls) {
   if (ls && !ls->active) return;
-  __shared__ double sh[kTB / kWave];
+  __shared__ double sh[kMaxTB / kWave];
   const int s = blockIdx.x;
   double hi = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += kTB) hi = fmax(hi, X[i + (long)s * n]);
+  for (int i = threadIdx.x; i < n; i += kMaxTB) hi = fmax(hi, X[i + (long)s * n]);
   for (int o = kWave / 2; o > 0; o /= 2) hi = fmax(hi, __shfl_xor(hi, o));
   if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = hi;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 1; i < kTB / kWave; ++i) hi = fmax(hi, sh[i]);
+    for (int i = 1; i < kMaxTB / kWave; ++i) hi = fmax(hi, sh[i]);
     m[s] = isfinite(hi) ? hi : 0.0;
   }
 }
@@ -147,7 +148,7 @@ template <class Op>
 void scaled_row(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
                 const double *F, const double *G, Op op, const LoopState *ls) {
   double *m = sp.m, *E = sp.E;
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, G, N, m, ls);
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, G, N, m, ls);
   check_launch();
   launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp((G[k] - m[k / N]) / reg); }, ls);
   SplitK<double> sk;
@@ -168,7 +169,7 @@ template <class Op>
 void scaled_col(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
                 const double *F, const double *G, Op op, const LoopState *ls) {
   double *m = sp.m, *E = sp.E, *mv = sp.mv;
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, F, M, m, ls);
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, F, M, m, ls);
   check_launch();
   launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp((F[k] - m[k / M]) / reg); }, ls);
   SplitK<double> sk;
@@ -209,7 +210,7 @@ void scaled_wcol(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, d
                  const double *F, const double *G, const double *rsh, const double *x, Op op) {
   double *m = sp.m, *E = sp.E, *mv = sp.mv;
   launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = (F[k] + rsh[k]) / reg; });
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, E, M, m, nullptr);
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, E, M, m, nullptr);
   check_launch();
   launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / M]) * x[k]; });
   SplitK<double> sk;
@@ -239,7 +240,7 @@ void scaled_wrow(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, d
                  const double *rowsh, Op op) {
   double *m = sp.m, *E = sp.E;
   launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = (G[k] + csh[k]) / reg; });
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kTB), 0, st, E, N, m, nullptr);
+  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, E, N, m, nullptr);
   check_launch();
   launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / N]) * y[k]; });
   SplitK<double> sk;

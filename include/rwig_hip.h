@@ -90,7 +90,10 @@ int rwig_barycenter_log(double *F, double *G, double *b, double *grad_A,
  * segment from its checkpoint, so the adjoint's memory is
  * max_iter/checkpoint_every + checkpoint_every + 2 slices of (M + N) x S
  * instead of max_iter + 1.
- *   precision 0: FP64 throughout (equal to rwig_barycenter_log);
+ *   precision 0: FP64 throughout (equal to rwig_barycenter_log up to
+ *                rounding); when (max C - min C) / reg <= 600 the soft-min
+ *                passes and the adjoint's weighted sums run as FP64 MFMA
+ *                products with the scaled kernel exp(-(C - min C) / reg);
  *   precision 1: FP32 cost, iterates and soft-min passes, FP64 gradient
  *                accumulation, loss and adjoint seed;
  *   precision 2: FP64 cost, iterates and exponent arguments, the passes'

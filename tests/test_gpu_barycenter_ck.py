@@ -47,6 +47,18 @@ def ck(A, C, w, bext, reg, L, **kw):
 @pytest.mark.parametrize("seed,M,N,S,reg", [(50, 40, 50, 3, 0.05), (51, 120, 100, 5, 0.02)])
 @pytest.mark.parametrize("every", [1, 7, 20, 1000])
 def test_fp64_reference_seed_matches_plain_and_oracle(seed, M, N, S, reg, every):
+    _reference_seed_case(seed, M, N, S, reg, every)
+
+
+@pytest.mark.parametrize("seed,M,N,S,reg", [(58, 60, 70, 2, 0.05), (59, 64, 64, 16, 0.05),
+                                            (60, 50, 40, 32, 0.03)])
+def test_fp64_slice_counts_match_plain_and_oracle(seed, M, N, S, reg):
+    # the 4-, 16- and 32-slice forms of the scaled passes' column update;
+    # M = N = 64 with a cost that is not symmetric (random points)
+    _reference_seed_case(seed, M, N, S, reg, 9)
+
+
+def _reference_seed_case(seed, M, N, S, reg, every):
     A, C, w, bext = bary_problem(seed, M, N, S)
     r = rwig_amd.barycenter_log_ck(A, C, w, bext, reg=reg, with_grad=True, max_iter=1000,
                                    zero_tol=1e-6, checkpoint_every=every)

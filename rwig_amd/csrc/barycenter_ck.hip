@@ -48,8 +48,8 @@ namespace {
 //
 // With FP64 iterates and a cost whose range (max C - min C) / reg is at most
 // kScaledRange, the row soft-min of R_s(i,j) = C(i,j) - F(i,s) - G(j,s) is
-//   -F(i,s) - reg log(sum_j Kt(i,j) E(j,s)) - m_s + cmin,
-// Kt = exp(-(C - cmin) / reg), E = exp((G - m_s) / reg), m_s = max_j G(j,s)
+//   -F(i,s) - reg (log(sum_j Kt(i,j) E(j,s)) + m_s) + cmin,
+// Kt = exp(-(C - cmin) / reg), E = exp(G / reg - m_s), m_s = max_j G(j,s) / reg
 // (the column soft-min likewise, with F and G exchanged): one FP64 MFMA
 // product of the M x N scaled kernel with the S slices per pass, in place of
 // an exponential per entry and slice. Every factor is at most 1, and in every
@@ -109,23 +109,46 @@ __global__ void __launch_bounds__(kTB) k_differ(const double *x, const double *y
   if (__any(d) && (threadIdx.x % kWave) == 0) atomicOr(flag, 1);
 }
 
-// m[s] = max_i X(i, s) over n rows (0 where no entry is finite), one block
-// of 1024 threads per slice
-constexpr int kMaxTB = 1024;
+constexpr int kMaxTB = 1024;  // threads of the per-slice operand kernel
+
+// The slice operand of a scaled pass, one block of 1024 threads per slice s:
+// x(i) = arg(i + s n) for i < n, m[s] = max_i x(i) (0 where no x is finite),
+// E[i + s n] = exp(x(i) - m[s]) * wt(i + s n); the arguments are staged in E
+// between the two sweeps.
+template <class Arg, class Wt>
 __global__ void __launch_bounds__(kMaxTB)
-k_slice_max(const double *X, int n, double *m, const LoopState *ls) {
+k_slice_operand(int n, Arg arg, Wt wt, double *E, double *m, const LoopState *ls) {
   if (ls && !ls->active) return;
   __shared__ double sh[kMaxTB / kWave];
+  __shared__ double msh;
   const int s = blockIdx.x;
+  double *e = E + (long)s * n;
   double hi = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += kMaxTB) hi = fmax(hi, X[i + (long)s * n]);
+  for (int i = threadIdx.x; i < n; i += kMaxTB) {
+    const double x = arg((long)i + (long)s * n);
+    e[i] = x;
+    hi = fmax(hi, x);
+  }
   for (int o = kWave / 2; o > 0; o /= 2) hi = fmax(hi, __shfl_xor(hi, o));
   if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = hi;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kMaxTB / kWave; ++i) hi = fmax(hi, sh[i]);
-    m[s] = isfinite(hi) ? hi : 0.0;
+    msh = isfinite(hi) ? hi : 0.0;
+    m[s] = msh;
   }
+  __syncthreads();
+  const double ms = msh;
+  for (int i = threadIdx.x; i < n; i += kMaxTB)
+    e[i] = exp(e[i] - ms) * wt((long)i + (long)s * n);
+}
+
+template <class Arg, class Wt>
+void slice_operand(hipStream_t st, int n, int S, Arg arg, Wt wt, double *E, double *m,
+                   const LoopState *ls) {
+  hipLaunchKernelGGL((k_slice_operand<Arg, Wt>), dim3(S), dim3(kMaxTB), 0, st, n, arg, wt, E, m,
+                     ls);
+  check_launch();
 }
 
 // The scaled kernel in both layouts and the passes' scratch.
@@ -148,9 +171,9 @@ template <class Op>
 void scaled_row(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
                 const double *F, const double *G, Op op, const LoopState *ls) {
   double *m = sp.m, *E = sp.E;
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, G, N, m, ls);
-  check_launch();
-  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp((G[k] - m[k / N]) / reg); }, ls);
+  slice_operand(
+      st, N, S, [=] __device__(long k) { return G[k] / reg; },
+      [] __device__(long) { return 1.0; }, E, m, ls);
   SplitK<double> sk;
   sk.ks = sp.ksr;
   sk.work = sk.ks > 1 ? sp.wr : nullptr;
@@ -159,7 +182,7 @@ void scaled_row(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, do
                             [=] __device__(int i, int s, double y) {
                               if (ls && !ls->active) return;
                               const long k = i + (long)s * M;
-                              op(k, -F[k] - reg * log(y) - m[s] + cm);
+                              op(k, -F[k] - reg * (log(y) + m[s]) + cm);
                             },
                             nullptr, nullptr, sk);
 }
@@ -169,9 +192,9 @@ template <class Op>
 void scaled_col(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
                 const double *F, const double *G, Op op, const LoopState *ls) {
   double *m = sp.m, *E = sp.E, *mv = sp.mv;
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, F, M, m, ls);
-  check_launch();
-  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp((F[k] - m[k / M]) / reg); }, ls);
+  slice_operand(
+      st, M, S, [=] __device__(long k) { return F[k] / reg; },
+      [] __device__(long) { return 1.0; }, E, m, ls);
   SplitK<double> sk;
   sk.ks = sp.ksc;
   sk.work = sk.ks > 1 ? sp.wc : nullptr;
@@ -180,7 +203,7 @@ void scaled_col(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, do
                             [=] __device__(int j, int s, double z) {
                               if (ls && !ls->active) return;
                               const long k = j + (long)s * N;
-                              mv[k] = -G[k] - reg * log(z) - m[s] + cm;
+                              mv[k] = -G[k] - reg * (log(z) + m[s]) + cm;
                             },
                             nullptr, nullptr, sk);
   RWIG_DISPATCH_S(S, SM,
@@ -209,10 +232,9 @@ template <class Op>
 void scaled_wcol(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, double reg,
                  const double *F, const double *G, const double *rsh, const double *x, Op op) {
   double *m = sp.m, *E = sp.E, *mv = sp.mv;
-  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = (F[k] + rsh[k]) / reg; });
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, E, M, m, nullptr);
-  check_launch();
-  launch_map(st, (long)M * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / M]) * x[k]; });
+  slice_operand(
+      st, M, S, [=] __device__(long k) { return (F[k] + rsh[k]) / reg; },
+      [=] __device__(long k) { return x[k]; }, E, m, nullptr);
   SplitK<double> sk;
   sk.ks = sp.ksc;
   sk.work = sk.ks > 1 ? sp.wc : nullptr;
@@ -239,10 +261,9 @@ void scaled_wrow(hipStream_t st, const ScaledSoftmin &sp, int M, int N, int S, d
                  const double *F, const double *G, const double *csh, const double *y,
                  const double *rowsh, Op op) {
   double *m = sp.m, *E = sp.E;
-  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = (G[k] + csh[k]) / reg; });
-  hipLaunchKernelGGL(k_slice_max, dim3(S), dim3(kMaxTB), 0, st, E, N, m, nullptr);
-  check_launch();
-  launch_map(st, (long)N * S, [=] __device__(long k) { E[k] = exp(E[k] - m[k / N]) * y[k]; });
+  slice_operand(
+      st, N, S, [=] __device__(long k) { return (G[k] + csh[k]) / reg; },
+      [=] __device__(long k) { return y[k]; }, E, m, nullptr);
   SplitK<double> sk;
   sk.ks = sp.ksr;
   sk.work = sk.ks > 1 ? sp.wr : nullptr;

@@ -189,6 +189,17 @@ __device__ __forceinline__ void tiny_row_pass(const double *__restrict__ A, int 
   auto dot = [&](int r, int c0, int c1) {
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     int c = c0;
+    // 16 loads in flight per thread where the range allows (the pass is
+    // bound by the latency of L2 reads), then blocks of 8: element c always
+    // goes to s[(c - c0) % 4], in order, so the sum does not depend on the
+    // block size
+    for (; c + 15 < c1; c += 16) {
+      double k[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) k[q] = A[r + (long)(c + q) * R];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
+    }
     for (; c + 7 < c1; c += 8) {
       double k[8];
 #pragma unroll

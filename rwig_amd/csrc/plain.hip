@@ -11,6 +11,8 @@
 // records, beside the iterates, the products the adjoint needs (K v_l,
 // K^T u_l, the soft-min vectors of every iteration), so each backward step is
 // two passes over the matrix instead of the reference's four or five.
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -183,32 +185,36 @@ __device__ __forceinline__ double tiny_sum(double x, double *red) {
 // else rows looped. Each thread keeps four accumulators and unrolls its loads
 // so that many are in flight: the pass is L2-latency-bound, not
 // bandwidth-bound, at these sizes.
+// sum_{c0 <= c < c1} a(c) x[c] with four accumulators: element c goes to
+// s[(c - c0) % 4] in order, whatever block of loads it falls in (16 in
+// flight where the range allows, then 8, then one at a time).
+template <class Acc>
+__device__ __forceinline__ double tiny_dot(Acc a, const double *x, int c0, int c1) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = c0;
+  for (; c + 15 < c1; c += 16) {
+    double k[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) k[q] = a(c + q);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
+  }
+  for (; c + 7 < c1; c += 8) {
+    double k[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) k[q] = a(c + q);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
+  }
+  for (; c < c1; ++c) s[0] = fma(a(c), x[c], s[0]);
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 __device__ __forceinline__ void tiny_row_pass(const double *__restrict__ A, int R, int Cn,
                                               const double *x, double *out, double *part) {
   const int tid = threadIdx.x;
   auto dot = [&](int r, int c0, int c1) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    int c = c0;
-    // 16 loads in flight per thread where the range allows (the pass is
-    // bound by the latency of L2 reads), then blocks of 8: element c always
-    // goes to s[(c - c0) % 4], in order, so the sum does not depend on the
-    // block size
-    for (; c + 15 < c1; c += 16) {
-      double k[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) k[q] = A[r + (long)(c + q) * R];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
-    }
-    for (; c + 7 < c1; c += 8) {
-      double k[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) k[q] = A[r + (long)(c + q) * R];  // 8 loads in flight
-#pragma unroll
-      for (int q = 0; q < 8; ++q) s[q & 3] = fma(k[q], x[c + q], s[q & 3]);
-    }
-    for (; c < c1; ++c) s[0] = fma(A[r + (long)c * R], x[c], s[0]);
-    return (s[0] + s[1]) + (s[2] + s[3]);
+    return tiny_dot([&](int c) { return A[r + (long)c * R]; }, x, c0, c1);
   };
   __syncthreads();  // A and x complete
   if (R <= kTinyTB) {
@@ -325,6 +331,187 @@ k_sinkhorn_vanilla_tiny(const double *__restrict__ a, const double *__restrict__
   }
 }
 
+// The same loop over G workgroups (a cooperative launch, G in 4 .. 64):
+// workgroup w keeps rows w pm .. of K and w pn .. of K^T in LDS, and u, v
+// and the error terms cross between workgroups through global memory at
+// grid-wide barriers, two per iteration: the next u is computed before the
+// barrier that publishes the error terms, and kept only if the loop goes on. One CU reads its rows of K from L2
+// at about 60 GB/s; G CUs hold theirs in LDS. Every value is computed as in
+// k_sinkhorn_vanilla_tiny: the same column groups and accumulators per row
+// (tiny_dot), u = a / K v and v = b / K^T u, and the two error sums over the
+// same 64-lane trees, so both forms give the same bits.
+constexpr int kCoopTB = 256;
+constexpr int kCoopWaves = kCoopTB / kWave;
+
+struct CoopBufs {
+  double *u, *v, *t1, *t2;  // u (M), v (N), error terms (2 M, 2 N: by iteration parity)
+};
+
+// LDS of the cooperative loop, in doubles
+inline size_t coop_lds(int M, int N, int G) {
+  const int pm = (M + G - 1) / G, pn = (N + G - 1) / G;
+  const int GM = kTinyTB / M, GN = kTinyTB / N;
+  return (size_t)pm * N + (size_t)pn * M + std::max(M, N) + 3 * (size_t)pm + 2 * (size_t)pn +
+         std::max((size_t)pm * GM, (size_t)pn * GN);
+}
+
+__global__ void __launch_bounds__(kCoopTB)
+k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__ b, int M, int N,
+                        int maxiter, double tol, const double *__restrict__ K,
+                        const double *__restrict__ KT, double *__restrict__ u,
+                        double *__restrict__ v, LoopState *ls, double *uh, double *vh,
+                        double *yh, double *zh, CoopBufs xb, int pm, int pn) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  extern __shared__ double lds[];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m0 = w * pm, n0 = w * pn;
+  const int mr = max(0, min(M, m0 + pm) - m0), nr = max(0, min(N, n0 + pn) - n0);
+  double *Ks = lds;                    // Ks[rl + c pm] = K(m0 + rl, c)
+  double *KTs = Ks + (size_t)pm * N;   // KTs[rl + c pn] = K^T(n0 + rl, c)
+  double *xs = KTs + (size_t)pn * M;   // the vector a pass multiplies
+  double *kv = xs + max(M, N), *uo = kv + pm, *un = uo + pm, *zo = un + pm, *vo = zo + pn;
+  double *part = vo + pn;              // column-group partials of a pass
+  __shared__ double red[kTinyWaves];
+  for (long k = tid; k < (long)pm * N; k += kCoopTB) {
+    const int rl = (int)(k % pm), c = (int)(k / pm);
+    Ks[k] = rl < mr ? K[m0 + rl + (long)c * M] : 0.0;
+  }
+  for (long k = tid; k < (long)pn * M; k += kCoopTB) {
+    const int rl = (int)(k % pn), c = (int)(k / pn);
+    KTs[k] = rl < nr ? KT[n0 + rl + (long)c * N] : 0.0;
+  }
+  // own rows of a pass over an R x Cn matrix held as S[rl + c p]: the column
+  // groups of tiny_row_pass (G = kTinyTB / R of width ceil(Cn / G)), their
+  // partials added in group order
+  auto pass = [&](const double *S, int p, int rows, int R, int Cn, double *out) {
+    const int G = kTinyTB / R, cw = (Cn + G - 1) / G;
+    __syncthreads();  // xs complete
+    for (int t = tid; t < rows * G; t += kCoopTB) {
+      const int rl = t % rows, g = t / rows;
+      part[g * p + rl] = tiny_dot([&](int c) { return S[rl + c * p]; }, xs, min(Cn, g * cw),
+                                  min(Cn, (g + 1) * cw));
+    }
+    __syncthreads();
+    for (int rl = tid; rl < rows; rl += kCoopTB) {
+      double acc = 0.0;
+      for (int q = 0; q < G; ++q) acc += part[q * p + rl];
+      out[rl] = acc;
+    }
+    __syncthreads();
+  };
+  // tiny_sum over a block of kTinyTB threads whose thread i holds terms[i]
+  // (0 past n): the same 64-lane trees, then the waves in order
+  auto tree_sum = [&](const double *terms, int n) {
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    for (int k = wv; k < kTinyWaves; k += kCoopWaves) {
+      const int i = k * kWave + lane;
+      const double x = wave_sum(i < n ? terms[i] : 0.0);
+      if (lane == 0) red[k] = x;
+    }
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < kTinyWaves; ++k) t += red[k];
+    __syncthreads();
+    return t;
+  };
+  for (int j = tid; j < N; j += kCoopTB) xs[j] = 1.0;
+  pass(Ks, pm, mr, M, N, kv);  // K v, v = 1
+  for (int rl = tid; rl < mr; rl += kCoopTB) uo[rl] = 1.0;
+  for (int rl = tid; rl < nr; rl += kCoopTB) vo[rl] = 1.0;
+  if (uh) {
+    for (int rl = tid; rl < mr; rl += kCoopTB) {
+      uh[m0 + rl] = 1.0;
+      yh[m0 + rl] = kv[rl];
+    }
+    for (int rl = tid; rl < nr; rl += kCoopTB) vh[n0 + rl] = 1.0;
+  }
+  int iter = 0;
+  double err = 1000.0;  // inst/include/sinkhorn_impl.hpp:85-88
+  bool active = iter < maxiter && err >= tol;
+  if (active) {  // u of the first iteration
+    for (int rl = tid; rl < mr; rl += kCoopTB) {
+      const int i = m0 + rl;
+      const double ui = a[i] / kv[rl];
+      uo[rl] = ui;
+      xb.u[i] = ui;
+      if (uh) uh[M + i] = ui;
+    }
+  }
+  grid.sync();
+  while (active) {
+    const long l = iter + 1;
+    double *t1 = xb.t1 + (l & 1) * M, *t2 = xb.t2 + (l & 1) * N;
+    for (int i = tid; i < M; i += kCoopTB) xs[i] = xb.u[i];
+    pass(KTs, pn, nr, N, M, zo);  // z = K^T u
+    for (int rl = tid; rl < nr; rl += kCoopTB) {
+      const int j = n0 + rl;
+      const double z = zo[rl], vj = b[j] / z;
+      vo[rl] = vj;
+      xb.v[j] = vj;
+      const double r = vj * z - b[j];
+      t2[j] = r * r;
+      if (uh) {
+        zh[l * N + j] = z;
+        vh[l * N + j] = vj;
+      }
+    }
+    grid.sync();
+    for (int j = tid; j < N; j += kCoopTB) xs[j] = xb.v[j];
+    pass(Ks, pm, mr, M, N, kv);  // K v
+    for (int rl = tid; rl < mr; rl += kCoopTB) {
+      const int i = m0 + rl;
+      const double r = uo[rl] * kv[rl] - a[i];
+      t1[i] = r * r;
+      if (uh) yh[l * M + i] = kv[rl];
+      const double ui = a[i] / kv[rl];  // the next iteration's u, if there is one
+      un[rl] = ui;
+      xb.u[i] = ui;
+    }
+    grid.sync();
+    const double e1 = tree_sum(t1, M), e2 = tree_sum(t2, N);
+    ++iter;
+    err = sqrt(e1) + sqrt(e2);
+    active = iter < maxiter && err >= tol;
+    if (active)
+      for (int rl = tid; rl < mr; rl += kCoopTB) {
+        uo[rl] = un[rl];
+        if (uh) uh[(iter + 1) * (long)M + m0 + rl] = un[rl];
+      }
+  }
+  for (int rl = tid; rl < mr; rl += kCoopTB) u[m0 + rl] = iter > 0 ? uo[rl] : 1.0;
+  for (int rl = tid; rl < nr; rl += kCoopTB) v[n0 + rl] = vo[rl];
+  if (w == 0 && tid == 0) {
+    ls->iter = iter;
+    ls->err = err;
+    ls->active = 0;
+    ls->maxiter = maxiter;
+    ls->tol = tol;
+  }
+}
+
+// Workgroups of the cooperative loop for an M x N problem (0: not this form):
+// both sides at most kTinyTB (the tiny loop's column groups), the smallest
+// G >= 4 whose LDS fits, and cooperative launches on the device.
+int coop_groups(int M, int N) {
+  if (M > kTinyTB || N > kTinyTB) return 0;
+  int dev = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+      !coop)
+    return 0;
+  for (int G = 4; G <= 64; G *= 2)
+    if (coop_lds(M, N, G) * sizeof(double) <= 150 * 1024) return G;
+  return 0;
+}
+
+// Which form runs a small vanilla problem: RWIG_VANILLA_FORM = "tiny" keeps
+// the single-workgroup loop (for comparisons); anything else lets the
+// cooperative loop run where it applies.
+bool coop_allowed() {
+  const char *f = std::getenv("RWIG_VANILLA_FORM");
+  return !(f && std::string(f) == "tiny");
+}
+
 // All pointers are device pointers of the reduced (support) problem.
 // u (M), v (N), P (M*N), loss (1), grad (M, withgrad only) are outputs.
 SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const double *b,
@@ -355,9 +542,35 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
     hipLaunchKernelGGL(k_gibbs_both, dim3((M + 31) / 32, (N + 31) / 32), dim3(1024), 0, st, C,
                        M, N, reg, K, KTb.get());
     check_launch();
-    hipLaunchKernelGGL(k_sinkhorn_vanilla_tiny, dim3(1), dim3(kTinyTB), lds, st, a, b, C, M, N,
-                       reg, maxiter, tol, K, KTb.get(), u, v, ls, uh, vh, yh, zh);
-    check_launch();
+    const int G = coop_allowed() ? coop_groups(M, N) : 0;
+    bool done = false;
+    if (G > 0) {
+      const int pm = (M + G - 1) / G, pn = (N + G - 1) / G;
+      const size_t clds = coop_lds(M, N, G) * sizeof(double);
+      DBuf<double> xbuf(3 * (size_t)M + 3 * (size_t)N);
+      CoopBufs xb{xbuf.get(), xbuf.get() + M, xbuf.get() + M + N, xbuf.get() + 3 * M + N};
+      const double *KTp = KTb.get();
+      void *args[] = {(void *)&a,  (void *)&b,  (void *)&M,  (void *)&N,  (void *)&maxiter,
+                      (void *)&tol, (void *)&K,  (void *)&KTp, (void *)&u,  (void *)&v,
+                      (void *)&ls, (void *)&uh, (void *)&vh, (void *)&yh, (void *)&zh,
+                      (void *)&xb, (void *)&pm, (void *)&pn};
+      RWIG_HIP_CHECK(hipFuncSetAttribute((const void *)k_sinkhorn_vanilla_coop,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+      const hipError_t e = hipLaunchCooperativeKernel((const void *)k_sinkhorn_vanilla_coop,
+                                                      dim3(G), dim3(kCoopTB), args, clds, st);
+      if (e == hipSuccess) {
+        done = true;
+        d2h(&fin, ls, 1, st);
+        RWIG_HIP_CHECK(hipStreamSynchronize(st));  // xbuf is released after the loop
+      } else {
+        (void)hipGetLastError();  // the device cannot hold the G workgroups at once
+      }
+    }
+    if (!done) {
+      hipLaunchKernelGGL(k_sinkhorn_vanilla_tiny, dim3(1), dim3(kTinyTB), lds, st, a, b, C, M,
+                         N, reg, maxiter, tol, K, KTb.get(), u, v, ls, uh, vh, yh, zh);
+      check_launch();
+    }
     d2h(&fin, ls, 1, st);
     RWIG_HIP_CHECK(hipStreamSynchronize(st));
   } else {

@@ -203,3 +203,24 @@ def test_c1_vanilla_256_config():
     assert r["iter"] == o["iter"]
     for k in ["P", "u", "v", "loss", "grad_a"]:
         assert rel(r[k], o[k]) < TOL_FP64, k
+
+
+@pytest.mark.parametrize("seed,M,N,zeros,reg,wg", [
+    (20, 256, 256, 0, 0.05, True),   # C1's size: 8 workgroups
+    (21, 300, 257, 10, 0.05, True),  # ragged rows per workgroup, zero-mass entries
+    (22, 40, 33, 3, 0.05, False),
+    (23, 1000, 64, 0, 0.02, False),  # the most workgroups
+])
+def test_vanilla_cooperative_loop_equals_the_one_workgroup_loop(seed, M, N, zeros, reg, wg,
+                                                                monkeypatch):
+    """Small vanilla problems run over several workgroups (cooperative launch,
+    K and K^T split over their LDS) with the one-workgroup loop's arithmetic:
+    the same bits."""
+    a, b, C = problem(seed, M, N, zeros)
+    ctrl = {"reg": reg, "method": "vanilla", "with_grad": wg}
+    r = rwig_amd.sinkhorn(a, b, C, ctrl)
+    monkeypatch.setenv("RWIG_VANILLA_FORM", "tiny")
+    t = rwig_amd.sinkhorn(a, b, C, ctrl)
+    assert r["iter"] == t["iter"] and r["err"] == t["err"]
+    for k in ["P", "u", "v", "loss"] + (["grad_a"] if wg else []):
+        assert np.array_equal(r[k], t[k]), k

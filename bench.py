@@ -82,7 +82,8 @@ def c1_leg(log, cpu=True, reps=50):
     """Plain vanilla Sinkhorn (C1): n = m = 256 on the 1-D grid, eps 0.05, up
     to 1000 iterations, stopping on the reference's error at 1e-9. The GPU
     form is the drop-in rwig_sinkhorn_vanilla (host buffers in and out; the
-    whole loop is one persistent workgroup); the CPU form the oracle's
+    whole loop is one launch of 8 workgroups holding K and K^T in LDS); the
+    CPU form the oracle's
     restatement of Sinkhorn::compute_vanilla on one thread (the reference
     runs BLAS single-threaded)."""
     import rwig_amd

@@ -11,7 +11,6 @@
 // records, beside the iterates, the products the adjoint needs (K v_l,
 // K^T u_l, the soft-min vectors of every iteration), so each backward step is
 // two passes over the matrix instead of the reference's four or five.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 
@@ -331,12 +330,15 @@ k_sinkhorn_vanilla_tiny(const double *__restrict__ a, const double *__restrict__
   }
 }
 
-// The same loop over G workgroups (a cooperative launch, G in 4 .. 64):
+// The same loop over G workgroups (G in 4 .. 64, one CU each):
 // workgroup w keeps rows w pm .. of K and w pn .. of K^T in LDS, and u, v
 // and the error terms cross between workgroups through global memory at
 // grid-wide barriers, two per iteration: the next u is computed before the
-// barrier that publishes the error terms, and kept only if the loop goes on. One CU reads its rows of K from L2
-// at about 60 GB/s; G CUs hold theirs in LDS. Every value is computed as in
+// barrier that publishes the error terms, and kept only if the loop goes on.
+// One CU reads its rows of K from L2 at about 60 GB/s; G CUs hold theirs in
+// LDS. The launch is an ordinary one with its own barrier (grid_barrier):
+// after one cooperative launch, the concurrent streams of a serving session
+// later in the same process ran a fifth slower. Every value is computed as in
 // k_sinkhorn_vanilla_tiny: the same column groups and accumulators per row
 // (tiny_dot), u = a / K v and v = b / K^T u, and the two error sums over the
 // same 64-lane trees, so both forms give the same bits.
@@ -345,7 +347,39 @@ constexpr int kCoopWaves = kCoopTB / kWave;
 
 struct CoopBufs {
   double *u, *v, *t1, *t2;  // u (M), v (N), error terms (2 M, 2 N: by iteration parity)
+  unsigned long long *bar;  // barrier arrivals, barrier failure flag (zero at launch)
 };
+
+// Polls of one barrier wait before it gives up (about a second)
+constexpr long kBarrierPolls = 1L << 24;
+
+// Grid-wide barrier of the G workgroups: the k-th barrier of the launch
+// completes when bar[0] reaches k G. Every thread releases its global writes
+// at agent scope before the workgroup barrier, one thread adds the
+// workgroup's arrival and polls, and every thread acquires after it. The
+// G workgroups need G CUs at once: they come up together on an idle device,
+// and otherwise as the kernels of other streams (which do not wait on this
+// one) leave their CUs. A wait is bounded: past kBarrierPolls polls it sets
+// bar[1], every later wait returns at once, the loop runs out, and the host
+// reports the failure.
+__device__ inline void grid_barrier(unsigned long long *bar, unsigned long long target) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&bar[0], 1ull);
+    long polls = 0;
+    while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++polls > kBarrierPolls) {
+        atomicExch(&bar[1], 1ull);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
 
 // LDS of the cooperative loop, in doubles
 inline size_t coop_lds(int M, int N, int G) {
@@ -361,8 +395,9 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
                         const double *__restrict__ KT, double *__restrict__ u,
                         double *__restrict__ v, LoopState *ls, double *uh, double *vh,
                         double *yh, double *zh, CoopBufs xb, int pm, int pn) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   extern __shared__ double lds[];
+  unsigned long long nbar = 0;  // barriers passed
+  auto grid_sync = [&] { grid_barrier(xb.bar, ++nbar * gridDim.x); };
   const int w = blockIdx.x, tid = threadIdx.x;
   const int m0 = w * pm, n0 = w * pn;
   const int mr = max(0, min(M, m0 + pm) - m0), nr = max(0, min(N, n0 + pn) - n0);
@@ -437,7 +472,7 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
       if (uh) uh[M + i] = ui;
     }
   }
-  grid.sync();
+  grid_sync();
   while (active) {
     const long l = iter + 1;
     double *t1 = xb.t1 + (l & 1) * M, *t2 = xb.t2 + (l & 1) * N;
@@ -455,7 +490,7 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
         vh[l * N + j] = vj;
       }
     }
-    grid.sync();
+    grid_sync();
     for (int j = tid; j < N; j += kCoopTB) xs[j] = xb.v[j];
     pass(Ks, pm, mr, M, N, kv);  // K v
     for (int rl = tid; rl < mr; rl += kCoopTB) {
@@ -467,7 +502,7 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
       un[rl] = ui;
       xb.u[i] = ui;
     }
-    grid.sync();
+    grid_sync();
     const double e1 = tree_sum(t1, M), e2 = tree_sum(t2, N);
     ++iter;
     err = sqrt(e1) + sqrt(e2);
@@ -490,15 +525,10 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
 }
 
 // Workgroups of the cooperative loop for an M x N problem (0: not this form):
-// both sides at most kTinyTB (the tiny loop's column groups), the smallest
-// G >= 4 whose LDS fits, and cooperative launches on the device.
+// both sides at most kTinyTB (the tiny loop's column groups), and the
+// smallest G >= 4 whose LDS fits.
 int coop_groups(int M, int N) {
   if (M > kTinyTB || N > kTinyTB) return 0;
-  int dev = 0, coop = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
-      !coop)
-    return 0;
   for (int G = 4; G <= 64; G *= 2)
     if (coop_lds(M, N, G) * sizeof(double) <= 150 * 1024) return G;
   return 0;
@@ -543,30 +573,27 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
                        M, N, reg, K, KTb.get());
     check_launch();
     const int G = coop_allowed() ? coop_groups(M, N) : 0;
-    bool done = false;
     if (G > 0) {
       const int pm = (M + G - 1) / G, pn = (N + G - 1) / G;
       const size_t clds = coop_lds(M, N, G) * sizeof(double);
       DBuf<double> xbuf(3 * (size_t)M + 3 * (size_t)N);
-      CoopBufs xb{xbuf.get(), xbuf.get() + M, xbuf.get() + M + N, xbuf.get() + 3 * M + N};
-      const double *KTp = KTb.get();
-      void *args[] = {(void *)&a,  (void *)&b,  (void *)&M,  (void *)&N,  (void *)&maxiter,
-                      (void *)&tol, (void *)&K,  (void *)&KTp, (void *)&u,  (void *)&v,
-                      (void *)&ls, (void *)&uh, (void *)&vh, (void *)&yh, (void *)&zh,
-                      (void *)&xb, (void *)&pm, (void *)&pn};
+      DBuf<unsigned long long> barb(2);
+      fill(st, barb.get(), 2, 0ull);
+      CoopBufs xb{xbuf.get(), xbuf.get() + M, xbuf.get() + M + N, xbuf.get() + 3 * M + N,
+                  barb.get()};
       RWIG_HIP_CHECK(hipFuncSetAttribute((const void *)k_sinkhorn_vanilla_coop,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
-      const hipError_t e = hipLaunchCooperativeKernel((const void *)k_sinkhorn_vanilla_coop,
-                                                      dim3(G), dim3(kCoopTB), args, clds, st);
-      if (e == hipSuccess) {
-        done = true;
-        d2h(&fin, ls, 1, st);
-        RWIG_HIP_CHECK(hipStreamSynchronize(st));  // xbuf is released after the loop
-      } else {
-        (void)hipGetLastError();  // the device cannot hold the G workgroups at once
-      }
-    }
-    if (!done) {
+      hipLaunchKernelGGL(k_sinkhorn_vanilla_coop, dim3(G), dim3(kCoopTB), clds, st, a, b, M, N,
+                         maxiter, tol, (const double *)K, (const double *)KTb.get(), u, v, ls,
+                         uh, vh, yh, zh, xb, pm, pn);
+      check_launch();
+      unsigned long long failed = 0;
+      d2h(&failed, barb.get() + 1, 1, st);
+      RWIG_HIP_CHECK(hipStreamSynchronize(st));  // xbuf is released after the loop
+      if (failed)
+        throw Error(RWIG_ERR_HIP, "vanilla loop: a grid barrier timed out (the " +
+                                      std::to_string(G) + " workgroups were not resident together)");
+    } else {
       hipLaunchKernelGGL(k_sinkhorn_vanilla_tiny, dim3(1), dim3(kTinyTB), lds, st, a, b, C, M,
                          N, reg, maxiter, tol, K, KTb.get(), u, v, ls, uh, vh, yh, zh);
       check_launch();

@@ -369,7 +369,10 @@ __device__ inline void grid_barrier(unsigned long long *bar, unsigned long long 
     atomicAdd(&bar[0], 1ull);
     long polls = 0;
     while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      // the failure flag every 256 polls, so that a poll is one load
+      if ((polls & 255) == 255 &&
+          __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        break;
       if (++polls > kBarrierPolls) {
         atomicExch(&bar[1], 1ull);
         break;

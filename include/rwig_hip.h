@@ -41,7 +41,11 @@ const char *rwig_last_error(void);
  * (inst/include/cuda_interface.cuh:13-18, src/cuda_sinkhorn.cu:181-329),
  * same argument order; semantics of Sinkhorn::compute_vanilla
  * (src/sinkhorn_impl_vanilla.cpp:18-86). grad_a is read only when withgrad.
- * return_status may be null. */
+ * return_status may be null. Reduced problems up to 1024 x 1024 run their
+ * whole loop in one launch over several workgroups that meet at grid-wide
+ * barriers; those workgroups must be resident at once, so the library holds
+ * up to 64 CUs for the loop's duration. A barrier wait is bounded (about a
+ * second): past it the call returns RWIG_ERR_HIP. */
 int rwig_sinkhorn_vanilla(double *P, double *grad_a, double *u, double *v,
                           double *loss, int *iter_out, double *err_out,
                           const double *a, const double *b, const double *C,

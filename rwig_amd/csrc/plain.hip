@@ -384,7 +384,7 @@ __device__ inline void grid_barrier(unsigned long long *bar, unsigned long long 
   __threadfence();
 }
 
-// LDS of the cooperative loop, in doubles
+// LDS of the multi-workgroup loop, in doubles
 inline size_t coop_lds(int M, int N, int G) {
   const int pm = (M + G - 1) / G, pn = (N + G - 1) / G;
   const int GM = kTinyTB / M, GN = kTinyTB / N;
@@ -527,7 +527,7 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
   }
 }
 
-// Workgroups of the cooperative loop for an M x N problem (0: not this form):
+// Workgroups of the multi-workgroup loop for an M x N problem (0: not this form):
 // both sides at most kTinyTB (the tiny loop's column groups), and the
 // smallest G >= 4 whose LDS fits.
 int coop_groups(int M, int N) {
@@ -539,7 +539,7 @@ int coop_groups(int M, int N) {
 
 // Which form runs a small vanilla problem: RWIG_VANILLA_FORM = "tiny" keeps
 // the single-workgroup loop (for comparisons); anything else lets the
-// cooperative loop run where it applies.
+// multi-workgroup loop run where it applies.
 bool coop_allowed() {
   const char *f = std::getenv("RWIG_VANILLA_FORM");
   return !(f && std::string(f) == "tiny");

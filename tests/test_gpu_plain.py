@@ -213,9 +213,9 @@ def test_c1_vanilla_256_config():
 ])
 def test_vanilla_cooperative_loop_equals_the_one_workgroup_loop(seed, M, N, zeros, reg, wg,
                                                                 monkeypatch):
-    """Small vanilla problems run over several workgroups (cooperative launch,
-    K and K^T split over their LDS) with the one-workgroup loop's arithmetic:
-    the same bits."""
+    """Small vanilla problems run over several workgroups (one launch, K and
+    K^T split over their LDS, grid-wide barriers through global memory) with
+    the one-workgroup loop's arithmetic: the same bits."""
     a, b, C = problem(seed, M, N, zeros)
     ctrl = {"reg": reg, "method": "vanilla", "with_grad": wg}
     r = rwig_amd.sinkhorn(a, b, C, ctrl)

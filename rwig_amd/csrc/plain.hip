@@ -527,6 +527,78 @@ k_sinkhorn_vanilla_coop(const double *__restrict__ a, const double *__restrict__
   }
 }
 
+// Steps l = L - 1 .. 1 of the vanilla adjoint (src/sinkhorn_impl_vanilla.cpp:
+// 169-214) over the same G workgroups and LDS slices as
+// k_sinkhorn_vanilla_coop: y = (-(K^T x) .* v_l) / (K^T u_l) over the
+// workgroup's columns, a grid barrier, then ubar = -K y, grad += ubar ./ (K
+// v_{l-1}) and x = ubar .* u_l ./ (K v_{l-1}) over its rows, a grid barrier.
+// x (M) and grad (M) hold step L's values on entry; y (N) is scratch. Each
+// workgroup writes only its own rows of x and grad and its own columns of y,
+// and reads x and y whole only between the barriers that publish them.
+__global__ void __launch_bounds__(kCoopTB)
+k_vanilla_adjoint_coop(int M, int N, int L, const double *__restrict__ K,
+                       const double *__restrict__ KT, const double *__restrict__ uh,
+                       const double *__restrict__ vh, const double *__restrict__ yh,
+                       const double *__restrict__ zh, double *x, double *y,
+                       double *__restrict__ grad, unsigned long long *bar, int pm, int pn) {
+  extern __shared__ double lds[];
+  unsigned long long nbar = 0;
+  auto grid_sync = [&] { grid_barrier(bar, ++nbar * gridDim.x); };
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m0 = w * pm, n0 = w * pn;
+  const int mr = max(0, min(M, m0 + pm) - m0), nr = max(0, min(N, n0 + pn) - n0);
+  double *Ks = lds;
+  double *KTs = Ks + (size_t)pm * N;
+  double *xs = KTs + (size_t)pn * M;
+  double *acc = xs + max(M, N);        // max(pm, pn) row results
+  double *part = acc + max(pm, pn);
+  for (long k = tid; k < (long)pm * N; k += kCoopTB) {
+    const int rl = (int)(k % pm), c = (int)(k / pm);
+    Ks[k] = rl < mr ? K[m0 + rl + (long)c * M] : 0.0;
+  }
+  for (long k = tid; k < (long)pn * M; k += kCoopTB) {
+    const int rl = (int)(k % pn), c = (int)(k / pn);
+    KTs[k] = rl < nr ? KT[n0 + rl + (long)c * N] : 0.0;
+  }
+  auto pass = [&](const double *S, int p, int rows, int R, int Cn) {
+    const int G = kTinyTB / R, cw = (Cn + G - 1) / G;
+    __syncthreads();
+    for (int t = tid; t < rows * G; t += kCoopTB) {
+      const int rl = t % rows, g = t / rows;
+      part[g * p + rl] = tiny_dot([&](int c) { return S[rl + c * p]; }, xs, min(Cn, g * cw),
+                                  min(Cn, (g + 1) * cw));
+    }
+    __syncthreads();
+    for (int rl = tid; rl < rows; rl += kCoopTB) {
+      double a = 0.0;
+      for (int q = 0; q < G; ++q) a += part[q * p + rl];
+      acc[rl] = a;
+    }
+    __syncthreads();
+  };
+  for (int l = L - 1; l > 0; --l) {
+    const double *ylm1 = yh + (long)(l - 1) * M, *ul = uh + (long)l * M;
+    const double *zl = zh + (long)l * N, *vl = vh + (long)l * N;
+    for (int i = tid; i < M; i += kCoopTB) xs[i] = x[i];
+    pass(KTs, pn, nr, N, M);  // K^T x
+    for (int rl = tid; rl < nr; rl += kCoopTB) {
+      const int j = n0 + rl;
+      const double vb = -acc[rl];
+      y[j] = (vb * vl[j]) / zl[j];
+    }
+    grid_sync();
+    for (int j = tid; j < N; j += kCoopTB) xs[j] = y[j];
+    pass(Ks, pm, mr, M, N);  // K y
+    for (int rl = tid; rl < mr; rl += kCoopTB) {
+      const int i = m0 + rl;
+      const double ub = -acc[rl];
+      grad[i] += ub / ylm1[i];
+      x[i] = (ub * ul[i]) / ylm1[i];
+    }
+    grid_sync();
+  }
+}
+
 // Workgroups of the multi-workgroup loop for an M x N problem (0: not this form):
 // both sides at most kTinyTB (the tiny loop's column groups), and the
 // smallest G >= 4 whose LDS fits.
@@ -568,14 +640,17 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
   double *uh = uhb.get(), *vh = vhb.get(), *yh = yhb.get(), *zh = zhb.get();
 
   LoopState fin{};
+  DBuf<double> KTb;  // K^T, kept for the adjoint of the multi-workgroup form
+  int Gv = 0;        // workgroups of that form (0: not run)
   if (tiny_vanilla(M, N)) {
-    DBuf<double> KTb(MN);
+    KTb.reset(MN);
     const size_t lds = sizeof(double) * (2 * (size_t)M + 2 * (size_t)N +
                                          std::max({kTinyTB, M, N}));
     hipLaunchKernelGGL(k_gibbs_both, dim3((M + 31) / 32, (N + 31) / 32), dim3(1024), 0, st, C,
                        M, N, reg, K, KTb.get());
     check_launch();
     const int G = coop_allowed() ? coop_groups(M, N) : 0;
+    Gv = G;
     if (G > 0) {
       const int pm = (M + G - 1) / G, pn = (N + G - 1) / G;
       const size_t clds = coop_lds(M, N, G) * sizeof(double);
@@ -707,6 +782,28 @@ SinkhornResult sinkhorn_vanilla_dev(hipStream_t st, const double *a, const doubl
               x[i] = (ub * ul[i]) / ylm1[i];
             },
             nullptr);
+      } else if (Gv > 0) {
+        // steps l .. 1 in one launch over the forward's workgroups
+        const int pm = (M + Gv - 1) / Gv, pn = (N + Gv - 1) / Gv;
+        const size_t clds = coop_lds(M, N, Gv) * sizeof(double);
+        DBuf<unsigned long long> barb(2);
+        fill(st, barb.get(), 2, 0ull);
+        RWIG_HIP_CHECK(hipFuncSetAttribute((const void *)k_vanilla_adjoint_coop,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)clds));
+        hipLaunchKernelGGL(k_vanilla_adjoint_coop, dim3(Gv), dim3(kCoopTB), clds, st, M, N,
+                           l + 1, (const double *)K, (const double *)KTb.get(),
+                           (const double *)uh, (const double *)vh, (const double *)yh,
+                           (const double *)zh, x, y, grad, barb.get(), pm, pn);
+        check_launch();
+        unsigned long long failed = 0;
+        d2h(&failed, barb.get() + 1, 1, st);
+        RWIG_HIP_CHECK(hipStreamSynchronize(st));
+        if (failed)
+          throw Error(RWIG_ERR_HIP, "vanilla adjoint: a grid barrier timed out (the " +
+                                        std::to_string(Gv) +
+                                        " workgroups were not resident together)");
+        break;
       } else {
         // vbar = -K^T x; y = (vbar .* v_l) / (K^T u_l)
         gemv_t(

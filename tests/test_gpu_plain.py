@@ -215,12 +215,17 @@ def test_vanilla_cooperative_loop_equals_the_one_workgroup_loop(seed, M, N, zero
                                                                 monkeypatch):
     """Small vanilla problems run over several workgroups (one launch, K and
     K^T split over their LDS, grid-wide barriers through global memory) with
-    the one-workgroup loop's arithmetic: the same bits."""
+    the one-workgroup loop's arithmetic: the same bits. The adjoint of the
+    multi-workgroup form sums its products in another order than the
+    multi-kernel adjoint, so grad_a agrees to 1e-12 relative."""
     a, b, C = problem(seed, M, N, zeros)
     ctrl = {"reg": reg, "method": "vanilla", "with_grad": wg}
     r = rwig_amd.sinkhorn(a, b, C, ctrl)
     monkeypatch.setenv("RWIG_VANILLA_FORM", "tiny")
     t = rwig_amd.sinkhorn(a, b, C, ctrl)
     assert r["iter"] == t["iter"] and r["err"] == t["err"]
-    for k in ["P", "u", "v", "loss"] + (["grad_a"] if wg else []):
+    for k in ["P", "u", "v", "loss"]:
         assert np.array_equal(r[k], t[k]), k
+    if wg:
+        g, h = r["grad_a"], t["grad_a"]
+        assert np.abs(g - h).max() <= 1e-12 * np.abs(h).max(), np.abs(g - h).max()
